@@ -1,0 +1,66 @@
+// Internal C-ABI between the host plans and the gfx950 kernels: plain
+// pointers, sizes and a stream. Every launcher returns the hipError_t of its
+// launch (hipSuccess on success).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// ---- TDD -----------------------------------------------------------------
+
+// Time-major packed input -> channel-major rows packed along time.
+// in: nsamps rows of in_stride bytes (channel c of row t at bit (c*nbits) of
+// the row, LSB first). out: nchans rows of row_words 32-bit words; sample t of
+// channel c sits at bits (t % (32/nbits))*nbits of word t/(32/nbits). Samples
+// t in [nsamps, row_words*32/nbits) are written as zero.
+hipError_t dd_tdd_transpose(const uint8_t* in, size_t nsamps, size_t in_stride,
+                            unsigned nbits, size_t nchans, uint32_t* out,
+                            size_t row_words, hipStream_t stream);
+
+// Epilogue constants of scale_output (src/tdd/dedisperse/dedisperse_kernel.cuh:45-64).
+typedef struct {
+    float out_range;  // 1 (float out) or 2^out_nbits - 1
+    float den;        // in_range * (float)nchans
+    float factor;     // (3.f*1024.f)/255.f/16.f
+    unsigned out_nbits;
+} dd_tdd_epilogue;
+
+// Generic channel sum: any nbits (1..32), integer killmask weights.
+// delays: [ndm][nact] integer sample delays; chan: [nact] channel indices;
+// weight: [nact] killmask values. out row i at out + i*out_stride bytes.
+hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row_words, unsigned nbits,
+                                     const uint32_t* delays, const int32_t* chan,
+                                     const int32_t* weight, size_t nact, size_t ndm,
+                                     size_t nsamps_out, uint8_t* out, size_t out_stride,
+                                     dd_tdd_epilogue ep, hipStream_t stream);
+
+// Tiling of the LDS (packed-lane) channel-sum kernel.
+typedef struct {
+    unsigned samples_per_lane;  // S: output samples one lane owns (multiple of 32/nbits)
+    unsigned dms_per_wave;      // DMs each wave accumulates
+    unsigned waves;             // waves per workgroup (DM block = waves*dms_per_wave)
+    unsigned chunk_channels;    // channels staged in LDS per pass
+    unsigned window_words;      // LDS words per staged channel
+} dd_tdd_tiling;
+
+// Packed-lane channel sum for nbits in {1,2,4,8} and a 0/1 killmask (only the
+// nact active channels are summed). wstart: [ndmblk][nact] first staged word
+// of each channel relative to the time tile's first word.
+hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_words, unsigned nbits,
+                                 const uint32_t* delays, const int32_t* chan,
+                                 const uint32_t* wstart, size_t nact, size_t ndm,
+                                 size_t nsamps_out, uint8_t* out, size_t out_stride,
+                                 dd_tdd_epilogue ep, dd_tdd_tiling tiling,
+                                 hipStream_t stream);
+
+// Whether dd_tdd_dedisperse_lds has an instantiation for this tiling.
+int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tiling);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,485 @@
+// gfx950 kernels of the time-domain dedispersion (TDD) path.
+//
+//  dd_tdd_transpose      time-major packed input -> channel-major rows packed
+//                        along time (one pass, LDS tile, zero fill of the tail)
+//  tdd_generic_k         one thread per (DM, output sample); any nbits and
+//                        integer killmask weights (exact uint32/uint64 sums)
+//  tdd_lds_k             the fast path: a wave owns one DM at a time, so every
+//                        per-(DM, channel) delay is wave-uniform (scalar
+//                        registers); delay-shifted channel windows are staged in
+//                        LDS; two aligned LDS words and one v_alignbit (or
+//                        v_perm for 8-bit) give a lane 32/nbits consecutive
+//                        samples; sums accumulate in packed 8-bit lanes, then
+//                        16-bit lanes, then 32-bit (SWAR), bit-exact to the
+//                        uint32 sums of the reference.
+//
+// Arithmetic restated from the reference: the integer delay
+// __float2uint_rn(dm*delay[c]) (src/tdd/dedisperse/dedisperse_kernel.cuh:163-165)
+// is precomputed on the host with the same float product and rounding; sums
+// and the output epilogue follow dedisperse_kernel.cuh:45-87, 151-193.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Epilogue (src/tdd/dedisperse/dedisperse_kernel.cuh:45-87)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float tdd_scale(float fsum, const dd_tdd_epilogue& ep)
+{
+    // (float)sum * out_range / (in_range * nchans) * factor, IEEE division
+    float num = fsum * ep.out_range;
+    float q = __fdiv_rn(num, ep.den);
+    return q * ep.factor;
+}
+
+__device__ __forceinline__ void tdd_store(uint8_t* row, size_t t, float scaled,
+                                          const dd_tdd_epilogue& ep)
+{
+    if (ep.out_nbits == 32) {
+        reinterpret_cast<float*>(row)[t] = scaled;
+    } else {
+        float v = fminf(fmaxf(scaled, 0.f), ep.out_range);  // clip, then truncate
+        if (ep.out_nbits == 16) reinterpret_cast<uint16_t*>(row)[t] = (uint16_t)__float2uint_rz(v);
+        else row[t] = (uint8_t)__float2uint_rz(v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transpose: time-major packed rows -> channel-major packed-along-time rows
+// ---------------------------------------------------------------------------
+
+constexpr int TR_TS = 256;  // time samples per tile
+constexpr int TR_CB = 64;   // channel bytes per tile
+constexpr int TR_PITCH = TR_CB + 4;
+
+template <int NBITS>
+__global__ __launch_bounds__(256) void tdd_transpose_k(
+    const uint8_t* __restrict__ in, size_t nsamps, size_t in_stride, size_t row_bytes,
+    size_t nchans, uint32_t* __restrict__ out, size_t row_words)
+{
+    constexpr int SPW = 32 / NBITS;                // samples per output word
+    constexpr int CH_PER_TILE = TR_CB * 8 / NBITS; // channels per tile
+    constexpr int WPC = TR_TS / SPW;               // output words per channel per tile
+    __shared__ uint8_t tile[TR_TS][TR_PITCH];
+
+    const size_t t0 = (size_t)blockIdx.x * TR_TS;
+    const size_t b0 = (size_t)blockIdx.y * TR_CB;
+    const int tid = threadIdx.x;
+
+    // Load TR_TS rows x TR_CB bytes (zero outside the input).
+    for (int i = tid; i < TR_TS * TR_CB; i += 256) {
+        int tl = i / TR_CB, bl = i % TR_CB;
+        size_t t = t0 + tl, b = b0 + bl;
+        uint8_t v = 0;
+        if (t < nsamps && b < row_bytes) v = in[t * in_stride + b];
+        tile[tl][bl] = v;
+    }
+    __syncthreads();
+
+    const size_t c0 = (size_t)blockIdx.y * CH_PER_TILE;
+    const size_t w0 = t0 / SPW;
+    for (int i = tid; i < CH_PER_TILE * WPC; i += 256) {
+        int cl = i / WPC, wl = i % WPC;
+        size_t c = c0 + cl;
+        if (c >= nchans || w0 + wl >= row_words) continue;
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            const uint8_t* r = tile[wl * SPW + j];
+            uint32_t s;
+            if (NBITS == 32) {
+                s = (uint32_t)r[4 * cl] | ((uint32_t)r[4 * cl + 1] << 8) |
+                    ((uint32_t)r[4 * cl + 2] << 16) | ((uint32_t)r[4 * cl + 3] << 24);
+            } else if (NBITS == 16) {
+                s = (uint32_t)r[2 * cl] | ((uint32_t)r[2 * cl + 1] << 8);
+            } else if (NBITS == 8) {
+                s = r[cl];
+            } else {
+                int bit = cl * NBITS;
+                s = (r[bit >> 3] >> (bit & 7)) & ((1u << NBITS) - 1u);
+            }
+            word |= (NBITS == 32) ? s : (s << (j * NBITS));
+        }
+        out[c * row_words + w0 + wl] = word;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic channel sum
+// ---------------------------------------------------------------------------
+
+template <int NBITS>
+__global__ __launch_bounds__(256) void tdd_generic_k(
+    const uint32_t* __restrict__ rows, size_t row_words,
+    const uint32_t* __restrict__ delays, const int32_t* __restrict__ chan,
+    const int32_t* __restrict__ weight, uint32_t nact, size_t nsamps_out,
+    uint8_t* __restrict__ out, size_t out_stride, dd_tdd_epilogue ep)
+{
+    constexpr int SPW = 32 / NBITS;
+    constexpr uint32_t MASK = (NBITS == 32) ? 0xffffffffu : ((1u << NBITS) - 1u);
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t i = blockIdx.y;
+    if (t >= nsamps_out) return;
+    const uint32_t* drow = delays + i * nact;
+    uint32_t sum32 = 0;
+    unsigned long long sum64 = 0;
+    for (uint32_t j = 0; j < nact; ++j) {
+        size_t ts = t + drow[j];
+        uint32_t w = rows[(size_t)chan[j] * row_words + ts / SPW];
+        uint32_t v = (NBITS == 32) ? w : ((w >> ((ts % SPW) * NBITS)) & MASK);
+        uint32_t term = (uint32_t)weight[j] * v;  // int * unsigned wraps mod 2^32
+        if (NBITS == 32) sum64 += term;
+        else sum32 += term;
+    }
+    float fsum = (NBITS == 32) ? (float)sum64 : (float)sum32;
+    tdd_store(out + i * out_stride, t, tdd_scale(fsum, ep), ep);
+}
+
+// ---------------------------------------------------------------------------
+// Packed-lane LDS channel sum
+// ---------------------------------------------------------------------------
+//
+// Tile: a workgroup owns 64*G*SPW consecutive output samples of WAVES*DPW DMs.
+// Lane l of a wave owns the G input words l, l+64, .., l+64(G-1) of the tile
+// (SPW samples each), so a wave's LDS reads are unit-stride.
+// code[dm][j] = (word offset of the delayed window inside the staged window)
+//               << 5 | bit shift (sub-byte) or byte offset (8-bit).
+//
+// Lane layout of the packed accumulators for one input word k:
+//   P = 8/NBITS byte-lane registers m (P = 1 for 8-bit); byte b of register m
+//   holds sample b*P + m of the word. acc16[k][m][h] holds the sums of bytes
+//   h (low half) and h+2 (high half), i.e. samples P*h+m and P*(h+2)+m.
+
+template <int NBITS, int G, int DPW, int WAVES, bool NEED32>
+__global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
+    const uint32_t* __restrict__ rows, size_t row_words,
+    const uint32_t* __restrict__ code, const int32_t* __restrict__ chan,
+    const uint32_t* __restrict__ wstart, uint32_t nact, uint32_t ndm,
+    uint32_t nsamps_out, uint32_t ndmblk, uint32_t cc_max, uint32_t W,
+    uint8_t* __restrict__ out, size_t out_stride, dd_tdd_epilogue ep)
+{
+    constexpr int SPW = 32 / NBITS;
+    constexpr int P = (NBITS < 8) ? 8 / NBITS : 1;
+    constexpr int TILE_WORDS = 64 * G;
+    constexpr uint32_t MAXV = (1u << NBITS) - 1u;
+    // byte lanes hold at most 255: flush every FL8 channels (even count)
+    constexpr uint32_t FL8 = (NBITS < 8) ? ((255u / MAXV) & ~1u) : 0xffffffffu;
+    constexpr uint32_t M8 = (NBITS == 4) ? 0x0F0F0F0Fu : (NBITS == 2) ? 0x03030303u : 0x01010101u;
+
+    extern __shared__ uint32_t lds[];
+
+    // XCD-aware mapping: workgroups are dispatched round-robin over the 8 XCDs;
+    // give each XCD a contiguous range of (time tile, DM block) pairs with the
+    // DM block fastest, so the blocks sharing a time tile's input run on the
+    // same L2.
+    const uint32_t nb = gridDim.x, bid = blockIdx.x;
+    const uint32_t xcd = bid & 7u, per = nb >> 3, rem = nb & 7u;
+    const uint32_t L = xcd * per + min(xcd, rem) + (bid >> 3);
+    const uint32_t tile = L / ndmblk, dblk = L % ndmblk;
+
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t tw0 = tile * TILE_WORDS;  // first input word of the tile
+    const uint32_t dmb = dblk * (WAVES * DPW) + wave * DPW;
+    uint32_t dmi[DPW];
+#pragma unroll
+    for (int i = 0; i < DPW; ++i) dmi[i] = min(dmb + i, ndm - 1);
+
+    uint32_t acc8[DPW][G][P];
+    uint32_t acc16[DPW][G][P][2];
+    uint32_t acc32[NEED32 ? DPW : 1][NEED32 ? G : 1][P][4];
+#pragma unroll
+    for (int i = 0; i < DPW; ++i)
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                acc8[i][k][m] = 0;
+                acc16[i][k][m][0] = acc16[i][k][m][1] = 0;
+            }
+    if constexpr (NEED32) {
+#pragma unroll
+        for (int i = 0; i < DPW; ++i)
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int m = 0; m < P; ++m)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc32[i][k][m][q] = 0;
+    }
+
+    // byte lanes -> 16-bit lanes: bytes (0,2) and (1,3) moved by one v_perm each
+    auto flush8 = [&]() {
+#pragma unroll
+        for (int i = 0; i < DPW; ++i)
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    uint32_t a = acc8[i][k][m];
+                    acc16[i][k][m][0] += __builtin_amdgcn_perm(0u, a, 0x0C020C00u);
+                    acc16[i][k][m][1] += __builtin_amdgcn_perm(0u, a, 0x0C030C01u);
+                    acc8[i][k][m] = 0;
+                }
+    };
+    auto flush16 = [&]() {
+        if constexpr (NEED32) {
+#pragma unroll
+            for (int i = 0; i < DPW; ++i)
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+#pragma unroll
+                    for (int m = 0; m < P; ++m)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            uint32_t a = acc16[i][k][m][h];
+                            acc32[i][k][m][h] += a & 0xFFFFu;
+                            acc32[i][k][m][h + 2] += a >> 16;
+                            acc16[i][k][m][h] = 0;
+                        }
+        }
+    };
+
+    // Adds the samples of one or two channels for DM i: each lane reads two
+    // aligned LDS words per input word k and funnel-shifts them to the
+    // delayed position (v_alignbit; v_perm for bytes). Two channels are summed
+    // before touching the accumulator, so the adds fuse into v_add3_u32.
+    auto extract = [&](const uint32_t* wbase, uint32_t shift, int k, int m) -> uint32_t {
+        const uint32_t* p = wbase + lane + 64 * k;
+        uint32_t w0 = p[0], w1 = p[1];
+        if constexpr (NBITS == 8) {
+            // byte offset -> selectors for bytes (r, r+2) and (r+1, r+3)
+            uint32_t sel = 0x0C000C00u + shift * 0x00010001u + 0x00020000u + (uint32_t)m * 0x00010001u;
+            return __builtin_amdgcn_perm(w1, w0, sel);
+        } else {
+            uint32_t x = __builtin_amdgcn_alignbit(w1, w0, shift);
+            return (x >> (m * NBITS)) & M8;
+        }
+    };
+    auto accumulate2 = [&](int i, const uint32_t* b0, uint32_t s0, const uint32_t* b1, uint32_t s1) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if constexpr (NBITS == 8) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    acc16[i][k][0][h] += extract(b0, s0, k, h) + extract(b1, s1, k, h);
+            } else {
+                const uint32_t* p0 = b0 + lane + 64 * k;
+                const uint32_t* p1 = b1 + lane + 64 * k;
+                uint32_t x0 = __builtin_amdgcn_alignbit(p0[1], p0[0], s0);
+                uint32_t x1 = __builtin_amdgcn_alignbit(p1[1], p1[0], s1);
+#pragma unroll
+                for (int m = 0; m < P; ++m)
+                    acc8[i][k][m] += ((x0 >> (m * NBITS)) & M8) + ((x1 >> (m * NBITS)) & M8);
+            }
+        }
+    };
+    auto accumulate1 = [&](int i, const uint32_t* b0, uint32_t s0) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if constexpr (NBITS == 8) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc16[i][k][0][h] += extract(b0, s0, k, h);
+            } else {
+#pragma unroll
+                for (int m = 0; m < P; ++m) acc8[i][k][m] += extract(b0, s0, k, m);
+            }
+        }
+    };
+
+    // Per-DM rows of the code table: wave-uniform 32-bit offsets (scalar loads).
+    uint32_t crow[DPW];
+#pragma unroll
+    for (int i = 0; i < DPW; ++i) crow[i] = __builtin_amdgcn_readfirstlane(dmi[i] * nact);
+    const uint32_t wrow = __builtin_amdgcn_readfirstlane(dblk * nact);
+
+    // channels per byte-lane group (sub-byte): the byte lanes are widened
+    // after every group; 8-bit input goes straight to 16-bit lanes
+    constexpr uint32_t GROUP = (NBITS < 8) ? FL8 : 0x7fffffffu;
+    for (uint32_t j0 = 0; j0 < nact; j0 += cc_max) {
+        const uint32_t cc = min(cc_max, nact - j0);
+        __syncthreads();
+        // Stage the delayed windows of channels j0 .. j0+cc-1.
+        for (uint32_t jj = wave; jj < cc; jj += WAVES) {
+            const uint32_t j = j0 + jj;
+            const uint32_t* src = rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
+            uint32_t* dst = lds + jj * W;
+            for (uint32_t w = lane; w < W; w += 64) dst[w] = src[w];
+        }
+        __syncthreads();
+
+        for (uint32_t g0 = 0; g0 < cc; g0 += GROUP) {
+            const uint32_t g1 = min(cc, g0 + GROUP);
+            uint32_t jj = g0;
+            for (; jj + 1 < g1; jj += 2) {
+                const uint32_t j = j0 + jj;
+                const uint32_t* l0 = lds + jj * W;
+                const uint32_t* l1 = l0 + W;
+#pragma unroll
+                for (int i = 0; i < DPW; ++i) {
+                    const uint32_t c0 = code[crow[i] + j], c1 = code[crow[i] + j + 1];
+                    accumulate2(i, l0 + (c0 >> 5), c0 & 31u, l1 + (c1 >> 5), c1 & 31u);
+                }
+            }
+            if (jj < g1) {
+                const uint32_t j = j0 + jj;
+                const uint32_t* l0 = lds + jj * W;
+#pragma unroll
+                for (int i = 0; i < DPW; ++i) {
+                    const uint32_t c0 = code[crow[i] + j];
+                    accumulate1(i, l0 + (c0 >> 5), c0 & 31u);
+                }
+            }
+            if constexpr (NBITS < 8) flush8();
+        }
+        flush16();
+    }
+
+    // Epilogue: lane l owns samples (l + 64k)*SPW + s of the tile.
+    const uint32_t t_tile = tile * TILE_WORDS * SPW;
+#pragma unroll
+    for (int i = 0; i < DPW; ++i) {
+        const uint32_t dm = dmb + i;
+        if (dm >= ndm) continue;
+        uint8_t* orow = out + (size_t)dm * out_stride;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const uint32_t tbase = t_tile + (uint32_t)(lane + 64 * k) * SPW;
+#pragma unroll
+            for (int s = 0; s < SPW; ++s) {
+                // sample s of word k: s = P*(h + 2*half) + m
+                const int m = s % P, q = s / P;   // q = h + 2*half
+                const int h = q & 1, half = q >> 1;
+                uint32_t sum;
+                if constexpr (NEED32) sum = acc32[i][k][m][h + 2 * half];
+                else sum = half ? (acc16[i][k][m][h] >> 16) : (acc16[i][k][m][h] & 0xFFFFu);
+                const uint32_t t = tbase + s;
+                if (t < nsamps_out) tdd_store(orow, t, tdd_scale((float)sum, ep), ep);
+            }
+        }
+    }
+}
+
+template <int NBITS, int G, int DPW, int WAVES, bool NEED32>
+hipError_t launch_lds(const uint32_t* rows, size_t row_words, const uint32_t* code,
+                      const int32_t* chan, const uint32_t* wstart, size_t nact, size_t ndm,
+                      size_t nsamps_out, uint8_t* out, size_t out_stride,
+                      dd_tdd_epilogue ep, const dd_tdd_tiling& tl, hipStream_t stream)
+{
+    constexpr int SPW = 32 / NBITS;
+    const size_t tile_samps = (size_t)64 * G * SPW;
+    const size_t ntiles = (nsamps_out + tile_samps - 1) / tile_samps;
+    const size_t ndmblk = (ndm + WAVES * DPW - 1) / (WAVES * DPW);
+    const size_t nblocks = ntiles * ndmblk;
+    const size_t lds_bytes = (size_t)tl.chunk_channels * tl.window_words * 4;
+    auto kern = tdd_lds_k<NBITS, G, DPW, WAVES, NEED32>;
+    if (lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(WAVES * 64), lds_bytes, stream,
+                       rows, row_words, code, chan, wstart, (uint32_t)nact, (uint32_t)ndm,
+                       (uint32_t)nsamps_out, (uint32_t)ndmblk, tl.chunk_channels,
+                       tl.window_words, out, out_stride, ep);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" hipError_t dd_tdd_transpose(const uint8_t* in, size_t nsamps, size_t in_stride,
+                                       unsigned nbits, size_t nchans, uint32_t* out,
+                                       size_t row_words, hipStream_t stream)
+{
+    const size_t row_bytes = (nchans * nbits + 7) / 8;
+    const size_t spw = 32 / nbits;
+    const size_t total_samps = row_words * spw;
+    dim3 grid((unsigned)((total_samps + TR_TS - 1) / TR_TS),
+              (unsigned)((row_bytes + TR_CB - 1) / TR_CB));
+    const size_t rb = (in_stride < row_bytes) ? in_stride : row_bytes;
+    switch (nbits) {
+    case 1:  hipLaunchKernelGGL(tdd_transpose_k<1>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 2:  hipLaunchKernelGGL(tdd_transpose_k<2>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 4:  hipLaunchKernelGGL(tdd_transpose_k<4>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 8:  hipLaunchKernelGGL(tdd_transpose_k<8>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 16: hipLaunchKernelGGL(tdd_transpose_k<16>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 32: hipLaunchKernelGGL(tdd_transpose_k<32>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row_words, unsigned nbits,
+                                                const uint32_t* delays, const int32_t* chan,
+                                                const int32_t* weight, size_t nact, size_t ndm,
+                                                size_t nsamps_out, uint8_t* out, size_t out_stride,
+                                                dd_tdd_epilogue ep, hipStream_t stream)
+{
+    if (nsamps_out == 0 || ndm == 0) return hipSuccess;
+    dim3 grid((unsigned)((nsamps_out + 255) / 256), (unsigned)ndm);
+#define DD_GENERIC(NB)                                                                         \
+    hipLaunchKernelGGL(tdd_generic_k<NB>, grid, dim3(256), 0, stream, rows, row_words, delays, \
+                       chan, weight, (uint32_t)nact, nsamps_out, out, out_stride, ep)
+    switch (nbits) {
+    case 1:  DD_GENERIC(1); break;
+    case 2:  DD_GENERIC(2); break;
+    case 4:  DD_GENERIC(4); break;
+    case 8:  DD_GENERIC(8); break;
+    case 16: DD_GENERIC(16); break;
+    case 32: DD_GENERIC(32); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef DD_GENERIC
+    return hipGetLastError();
+}
+
+// Instantiated tilings: (nbits, words per lane G, DMs per wave, waves).
+#define DD_LDS_TILINGS(X) \
+    X(1, 1, 2, 4) X(1, 1, 4, 4) \
+    X(2, 1, 2, 4) X(2, 2, 2, 4) X(2, 1, 4, 4) \
+    X(4, 2, 2, 4) X(4, 2, 4, 4) X(4, 1, 4, 4) X(4, 2, 2, 8) \
+    X(8, 2, 2, 4) X(8, 4, 2, 4) X(8, 2, 4, 4) X(8, 2, 2, 8)
+
+extern "C" int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tl)
+{
+    const unsigned spw = 32 / nbits;
+    if (tl.samples_per_lane % spw) return 0;
+    const unsigned G = tl.samples_per_lane / spw;
+#define DD_MATCH(NB, GG, DPW, WAVES) \
+    if (nbits == NB && G == GG && tl.dms_per_wave == DPW && tl.waves == WAVES) return 1;
+    DD_LDS_TILINGS(DD_MATCH)
+#undef DD_MATCH
+    return 0;
+}
+
+extern "C" hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_words, unsigned nbits,
+                                            const uint32_t* code, const int32_t* chan,
+                                            const uint32_t* wstart, size_t nact, size_t ndm,
+                                            size_t nsamps_out, uint8_t* out, size_t out_stride,
+                                            dd_tdd_epilogue ep, dd_tdd_tiling tl,
+                                            hipStream_t stream)
+{
+    if (nsamps_out == 0 || ndm == 0) return hipSuccess;
+    const unsigned spw = 32 / nbits;
+    const unsigned G = tl.samples_per_lane / spw;
+    const uint32_t maxv = (1u << nbits) - 1u;
+    // 16-bit lanes overflow beyond 65535: then sums are widened every staged chunk
+    const bool need32 = (uint64_t)nact * maxv > 65535u;
+#define DD_CALL(NB, GG, DPW, WAVES)                                                             \
+    if (nbits == NB && G == GG && tl.dms_per_wave == DPW && tl.waves == WAVES) {                \
+        if (need32)                                                                             \
+            return launch_lds<NB, GG, DPW, WAVES, true>(rows, row_words, code, chan, wstart,    \
+                                                        nact, ndm, nsamps_out, out, out_stride, \
+                                                        ep, tl, stream);                        \
+        return launch_lds<NB, GG, DPW, WAVES, false>(rows, row_words, code, chan, wstart, nact, \
+                                                     ndm, nsamps_out, out, out_stride, ep, tl,  \
+                                                     stream);                                   \
+    }
+    DD_LDS_TILINGS(DD_CALL)
+#undef DD_CALL
+    return hipErrorInvalidValue;
+}

@@ -1,0 +1,328 @@
+// TDDPlan: host side of the time-domain dedispersion path.
+//
+// Reference: src/tdd/TDDPlan.cpp:43-487. The reference streams 65536-sample
+// gulps through pinned double buffers because its device copy of the input is
+// gulp-sized; here the whole series is made resident in HBM (288 GB per GPU),
+// transposed once, and every output sample of every DM is produced by one
+// launch of the channel-sum kernel. The gulp size is kept as plan state for API
+// compatibility (set_gulp_size / get_gulp_size) and bounds the host-pointer
+// staging chunk.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+#include "internal.hpp"
+#include "kernels.h"
+
+namespace dedisp {
+
+struct TDDState {
+    // device buffers, grown on demand and kept between executes
+    DeviceBuffer d_in;      // staged input (host-pointer mode)
+    DeviceBuffer d_rows;    // channel-major packed rows
+    DeviceBuffer d_out;     // staged output (host-pointer mode)
+    DeviceBuffer d_code;    // [ndm][nact] fast path: window offset | shift; generic: delay
+    DeviceBuffer d_chan;    // [nact] channel index
+    DeviceBuffer d_weight;  // [nact] killmask weight (generic path)
+    DeviceBuffer d_wstart;  // [ndmblk][nact] staged window start (fast path)
+
+    // cache key of the device tables
+    unsigned long version = ~0ul;
+    unsigned nbits = 0;
+    size_t first_dm = ~size_t(0), dm_count = 0, nsamps_out = ~size_t(0);
+
+    // derived from the tables
+    bool fast = false;
+    dd_tdd_tiling tiling{};
+    size_t nact = 0;
+    size_t dmax = 0;        // largest delay (samples) over the DMs in use
+    size_t max_wstart = 0;  // fast path
+    size_t row_words = 0;
+};
+
+namespace {
+
+// Default tiling per input width (words per lane x SPW samples, DMs per wave,
+// waves); the chunk size is fitted to the LDS budget below. Overridden by the
+// environment variable DEDISP_TDD_TILING="samples_per_lane,dms_per_wave,waves,chunk".
+dd_tdd_tiling default_tiling(unsigned nbits)
+{
+    dd_tdd_tiling t{};
+    t.waves = 4;
+    t.dms_per_wave = 2;
+    switch (nbits) {
+    case 1: t.samples_per_lane = 32; break;
+    case 2: t.samples_per_lane = 16; break;
+    case 4: t.samples_per_lane = 16; break;
+    default: t.samples_per_lane = 8; break;
+    }
+    t.chunk_channels = 64;
+    return t;
+}
+
+constexpr size_t kLdsBudget = 64 * 1024;  // bytes of LDS per workgroup
+
+}  // namespace
+
+TDDPlan::TDDPlan(size_type nchans, float_type dt, float_type f0, float_type df, int device_idx)
+    : GPUPlan(nchans, dt, f0, df, device_idx), m_state(new TDDState)
+{
+    if (nchans > kTddMaxChans) DD_THROW(DEDISP_NCHANS_EXCEEDS_LIMIT);
+    m_gulp_size = 65536;  // src/tdd/TDDPlan.cpp:71-74
+}
+
+TDDPlan::~TDDPlan() {}
+
+void TDDPlan::set_gulp_size(size_type gulp_size) { m_gulp_size = gulp_size; }
+
+void TDDPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
+                      byte_type* out, size_type out_nbits, unsigned flags)
+{
+    // src/tdd/TDDPlan.cpp:102-127: unpadded strides
+    dedisp_size out_bytes_per_sample = out_nbits / 8;
+    dedisp_size in_stride = m_nchans * in_nbits / 8;
+    dedisp_size out_stride = (nsamps - m_max_delay) * out_bytes_per_sample;
+    execute_adv(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, flags);
+}
+
+void TDDPlan::execute_adv(size_type nsamps, const byte_type* in, size_type in_nbits,
+                          size_type in_stride, byte_type* out, size_type out_nbits,
+                          size_type out_stride, unsigned flags)
+{
+    execute_guru(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, 0, m_dm_count, flags);
+}
+
+void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_nbits,
+                           size_type in_stride, byte_type* out, size_type out_nbits,
+                           size_type out_stride, dedisp_size first_dm_idx,
+                           dedisp_size dm_count, unsigned flags)
+{
+    // Validation in the order of src/tdd/TDDPlan.cpp:167-195 (the unsigned
+    // wrap of nsamps - max_delay in the stride test is kept, so nsamps <
+    // max_delay reports DEDISP_INVALID_STRIDE as the reference does).
+    dedisp_size out_bytes_per_sample = out_nbits / 8;
+    if (!in || !out) DD_THROW(DEDISP_INVALID_POINTER);
+    if (in_stride < m_nchans * in_nbits / 8 ||
+        out_stride < (nsamps - m_max_delay) * out_bytes_per_sample)
+        DD_THROW(DEDISP_INVALID_STRIDE);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    // src/dedisp/DedispPlan.cpp:137-139
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8 && in_nbits != 16 &&
+        in_nbits != 32)
+        DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    if (first_dm_idx > m_dm_count || dm_count > m_dm_count - first_dm_idx)
+        DD_THROW(DEDISP_UNKNOWN_ERROR);
+    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const size_t nsc = nsamps - m_max_delay;
+    if (nsc == 0 || dm_count == 0) return;
+
+    GPUContext& ctx = *m_ctx;
+    TDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    ctx.timer.reset(ctx.profiling);
+    ctx.profile = dedisp_amd_profile{};
+
+    const unsigned nbits = (unsigned)in_nbits;
+    const size_t spw = 32 / nbits;
+
+    // ---- device tables (cached per DM list / killmask / nbits / DM range) ----
+    if (st.version != m_tables_version || st.nbits != nbits || st.first_dm != first_dm_idx ||
+        st.dm_count != dm_count || st.nsamps_out != nsc) {
+        std::vector<int32_t> chan, weight;
+        bool binary = true;
+        for (size_t c = 0; c < m_nchans; ++c) {
+            int32_t k = h_killmask[c];
+            if (k == 0) continue;
+            if (k != 1) binary = false;
+            chan.push_back((int32_t)c);
+            weight.push_back(k);
+        }
+        const size_t nact = chan.size();
+        std::vector<uint32_t> d((size_t)dm_count * nact);
+        size_t dmax = 0;
+        for (size_t i = 0; i < dm_count; ++i) {
+            float dm = h_dm_list[first_dm_idx + i];
+            for (size_t j = 0; j < nact; ++j) {
+                uint32_t v = tdd_delay(dm, h_delay_table[chan[j]]);
+                d[i * nact + j] = v;
+                dmax = std::max<size_t>(dmax, v);
+            }
+        }
+
+        dd_tdd_tiling tl = default_tiling(nbits);
+        if (const char* env = std::getenv("DEDISP_TDD_TILING")) {
+            unsigned a, b, c, e;
+            if (std::sscanf(env, "%u,%u,%u,%u", &a, &b, &c, &e) == 4) {
+                tl.samples_per_lane = a; tl.dms_per_wave = b; tl.waves = c; tl.chunk_channels = e;
+            }
+        }
+        bool fast = binary && nact > 0 && nbits <= 8 && dd_tdd_lds_supported(nbits, tl) &&
+                    (uint64_t)dm_count * nact < (uint64_t(1) << 31);
+
+        std::vector<uint32_t> code, wstart;
+        size_t max_wstart = 0, W = 0;
+        if (fast) {
+            const size_t G = tl.samples_per_lane / spw;
+            const size_t dpb = (size_t)tl.waves * tl.dms_per_wave;
+            const size_t ndmblk = div_up(dm_count, dpb);
+            wstart.assign(ndmblk * nact, 0);
+            size_t span = 0;
+            for (size_t b = 0; b < ndmblk; ++b) {
+                const size_t i0 = b * dpb, i1 = std::min(dm_count, i0 + dpb);
+                for (size_t j = 0; j < nact; ++j) {
+                    uint32_t lo = ~0u, hi = 0;
+                    for (size_t i = i0; i < i1; ++i) {
+                        uint32_t q = d[i * nact + j] / (uint32_t)spw;
+                        lo = std::min(lo, q);
+                        hi = std::max(hi, q);
+                    }
+                    wstart[b * nact + j] = lo;
+                    max_wstart = std::max<size_t>(max_wstart, lo);
+                    span = std::max<size_t>(span, hi - lo);
+                }
+            }
+            W = span + 64 * G + 1;
+            // channels per staged chunk: LDS budget, and for 8-bit input the
+            // 16-bit lanes hold at most 257 channels of 255
+            size_t cc = std::min<size_t>(tl.chunk_channels, nact);
+            if (nbits == 8) cc = std::min<size_t>(cc, 256);
+            while (cc > 1 && cc * W * 4 > kLdsBudget) cc /= 2;
+            if (cc * W * 4 > kLdsBudget || span >= (1u << 26)) {
+                fast = false;
+            } else {
+                tl.chunk_channels = (unsigned)cc;
+                tl.window_words = (unsigned)W;
+                code.resize(d.size());
+                for (size_t i = 0; i < dm_count; ++i) {
+                    const size_t b = i / dpb;
+                    for (size_t j = 0; j < nact; ++j) {
+                        uint32_t v = d[i * nact + j];
+                        uint32_t rel = v / (uint32_t)spw - wstart[b * nact + j];
+                        uint32_t sh = (nbits == 8) ? (v % 4u) : (v % (uint32_t)spw) * nbits;
+                        code[i * nact + j] = (rel << 5) | sh;
+                    }
+                }
+            }
+        }
+
+        size_t row_words;
+        if (fast) {
+            const size_t tile_words = (size_t)64 * (tl.samples_per_lane / spw);
+            const size_t ntiles = div_up(nsc, tile_words * spw);
+            row_words = ntiles * tile_words + max_wstart + W;
+        } else {
+            row_words = (nsc + dmax) / spw + 2;
+        }
+        row_words = std::max(row_words, div_up(nsamps, spw));
+        row_words = round_up(row_words, 64);
+
+        st.d_chan.reserve(std::max<size_t>(nact, 1) * 4);
+        st.d_weight.reserve(std::max<size_t>(nact, 1) * 4);
+        if (nact) {
+            DD_HIP(hipMemcpyAsync(st.d_chan.ptr, chan.data(), nact * 4, hipMemcpyHostToDevice, stream));
+            DD_HIP(hipMemcpyAsync(st.d_weight.ptr, weight.data(), nact * 4, hipMemcpyHostToDevice, stream));
+        }
+        const std::vector<uint32_t>& tab = fast ? code : d;
+        st.d_code.reserve(std::max<size_t>(tab.size(), 1) * 4);
+        if (!tab.empty())
+            DD_HIP(hipMemcpyAsync(st.d_code.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
+        if (fast) {
+            st.d_wstart.reserve(wstart.size() * 4);
+            DD_HIP(hipMemcpyAsync(st.d_wstart.ptr, wstart.data(), wstart.size() * 4,
+                                  hipMemcpyHostToDevice, stream));
+        }
+        DD_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
+
+        st.version = m_tables_version;
+        st.nbits = nbits;
+        st.first_dm = first_dm_idx;
+        st.dm_count = dm_count;
+        st.nsamps_out = nsc;
+        st.fast = fast;
+        st.tiling = tl;
+        st.nact = nact;
+        st.dmax = dmax;
+        st.max_wstart = max_wstart;
+        st.row_words = row_words;
+    }
+
+    // ---- input ----
+    const size_t row_bytes = (m_nchans * nbits + 7) / 8;
+    const uint8_t* d_in;
+    if (device_ptrs) {
+        d_in = in;
+    } else {
+        const size_t width = std::min(in_stride, row_bytes);
+        st.d_in.reserve(nsamps * in_stride + 64);
+        ctx.timer.begin(StageTimer::H2D, stream);
+        DD_HIP(hipMemcpy2DAsync(st.d_in.ptr, in_stride, in, in_stride, width, nsamps,
+                                hipMemcpyHostToDevice, stream));
+        ctx.timer.end(StageTimer::H2D, stream);
+        d_in = st.d_in.as<uint8_t>();
+    }
+
+    // ---- transpose ----
+    st.d_rows.reserve(m_nchans * st.row_words * 4);
+    ctx.timer.begin(StageTimer::UNPACK, stream);
+    DD_HIP(dd_tdd_transpose(d_in, nsamps, in_stride, nbits, m_nchans, st.d_rows.as<uint32_t>(),
+                            st.row_words, stream));
+    ctx.timer.end(StageTimer::UNPACK, stream);
+
+    // ---- channel sum ----
+    uint8_t* d_out;
+    size_t d_out_stride;
+    if (device_ptrs) {
+        d_out = out;
+        d_out_stride = out_stride;
+    } else {
+        d_out_stride = nsc * out_bytes_per_sample;
+        st.d_out.reserve(dm_count * d_out_stride);
+        d_out = st.d_out.as<uint8_t>();
+    }
+    dd_tdd_epilogue ep;
+    {
+        // scale_output constants (src/tdd/dedisperse/dedisperse_kernel.cuh:50-58)
+        uint32_t in_max = (nbits == 32) ? 0xffffffffu : ((1u << nbits) - 1u);
+        float in_range = (float)in_max;
+        ep.out_range = (out_nbits == 32) ? 1.f : (float)((out_nbits == 16) ? 65535u : 255u);
+        ep.den = in_range * (float)m_nchans;
+        ep.factor = (3.f * 1024.f) / 255.f / 16.f;
+        ep.out_nbits = (unsigned)out_nbits;
+    }
+    ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+    if (st.fast) {
+        DD_HIP(dd_tdd_dedisperse_lds(st.d_rows.as<uint32_t>(), st.row_words, nbits,
+                                     st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
+                                     st.d_wstart.as<uint32_t>(), st.nact, dm_count, nsc, d_out,
+                                     d_out_stride, ep, st.tiling, stream));
+    } else {
+        DD_HIP(dd_tdd_dedisperse_generic(st.d_rows.as<uint32_t>(), st.row_words, nbits,
+                                         st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
+                                         st.d_weight.as<int32_t>(), st.nact, dm_count, nsc,
+                                         d_out, d_out_stride, ep, stream));
+    }
+    ctx.timer.end(StageTimer::DEDISPERSE, stream);
+
+    // ---- output ----
+    if (!device_ptrs) {
+        ctx.timer.begin(StageTimer::D2H, stream);
+        DD_HIP(hipMemcpy2DAsync(out, out_stride, d_out, d_out_stride, nsc * out_bytes_per_sample,
+                                dm_count, hipMemcpyDeviceToHost, stream));
+        ctx.timer.end(StageTimer::D2H, stream);
+    }
+    if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
+        DD_HIP(hipStreamSynchronize(stream));
+        ctx.timer.collect(ctx.profile);
+        ctx.profile.work_units = (double)dm_count * (double)nsc * (double)m_nchans;
+    }
+}
+
+}  // namespace dedisp
