@@ -11,6 +11,10 @@ HBM (DEDISP_DEVICE_POINTERS): the transpose kernel plus the channel-sum kernel.
 Value: DM-trials x output samples per second.
 
 Prints one JSON line. Besides the contract fields it carries:
+  fdd            the FDD path on 1024 ch x 2^20 8-bit samples, 2048 DMs, chunks
+                 of 2^16: float32 MFMA, bfloat16 three-product MFMA and float16
+                 MFMA contraction forms, each with its error against the FP64
+                 oracle and its matrix-core roofline
   roofline       HBM view of the channel-sum kernel (algorithmic bytes / launch
                  time measured with HIP events on the kernel's stream)
   valu_roofline  the integer-add view of the same kernel (channel-adds / s
@@ -103,12 +107,96 @@ def cpu_baseline(seconds_budget: float = 30.0):
     }
 
 
+MFMA_PEAK_T = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # dense TFLOP/s
+
+
+def fdd_leg(torch, dd, steps: int, warmup: int) -> dict:
+    """FDD on 1024 ch x 2^20 8-bit samples (N(128,20), five pulses), 2048 linear DM
+    trials 0..1023.5, overlap-save chunks of 2^16 samples, time-series output,
+    input and output in HBM. One line per contraction form with its error against
+    the FP64 oracle on a sample of DM rows."""
+    import oracle
+    g = torch.Generator(device="cuda")
+    g.manual_seed(99)
+    x = torch.randn((NSAMPS, NCH), generator=g, device="cuda", dtype=torch.float32)
+    s = torch.clamp(torch.round(x * 20.0 + 128.0), 0, 255)
+    del x
+    delays_np = dd.generate_delay_table(NCH, DT, F0, DF)
+    delays = torch.from_numpy(delays_np).cuda()
+    rng = np.random.default_rng(99)
+    cols = torch.arange(NCH, device="cuda")
+    pulse_dms = []
+    for _ in range(5):
+        dm = float(rng.uniform(50, 900))
+        pulse_dms.append(dm)
+        width = int(rng.integers(1, 17))
+        t0 = int(rng.integers(0, NSAMPS // 2))
+        d = torch.round(torch.tensor(np.float32(dm), device="cuda") * delays).to(torch.int64)
+        for w in range(width):
+            t = t0 + d + w
+            ok = t < NSAMPS
+            s[t[ok], cols[ok]] = torch.clamp(s[t[ok], cols[ok]] + 30, max=255)
+    d_in = s.to(torch.uint8).contiguous()
+    del s
+    dms = (np.arange(2048, dtype=np.float32) * np.float32(0.5)).astype(np.float32)
+    host_in = d_in.cpu().numpy()
+    sample_rows = [0, 700, 1400, 2047]
+    ref = oracle.fdd_timeseries(NCH, oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF)),
+                                dms[sample_rows], NSAMPS, 0, host_in)
+    out = {"workload": "fdd-1024ch-2^20x8bit-2048dm-nfft65536", "ndm": 2048, "nfft": 65536}
+    for name, prec in (("fp32", dd.FDD_FP32), ("bf16", dd.FDD_BF16), ("fp16", dd.FDD_FP16)):
+        plan = dd.Plan(NCH, DT, F0, DF, impl=dd.FDD)
+        plan.fdd_set_output(dd.FDD_TIMESERIES)
+        plan.fdd_set_segmented(True, 65536)
+        plan.fdd_set_precision(prec)
+        plan.set_dm_list(dms)
+        maxd = plan.max_delay
+        nsc = NSAMPS - maxd
+        d_out = torch.empty((2048, nsc), dtype=torch.float32, device="cuda")
+        plan.set_profiling(True)
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            assert plan.execute_raw(NSAMPS, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        prof = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert plan.execute_raw(NSAMPS, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+            prof.append(plan.profile())
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        got = d_out[sample_rows].cpu().numpy().astype(np.float64)
+        r = ref[:, :nsc]
+        rel = float(np.linalg.norm(got - r) / np.linalg.norm(r))
+        maxabs = float(np.max(np.abs(got - r)))
+        c_ms = float(np.mean([p["dedisperse_ms"] for p in prof]))
+        cmacs = prof[-1]["work_units"]
+        tflops = 8 * cmacs / (c_ms * 1e-3) / 1e12
+        out[name] = {
+            "value": 2048 * nsc / el, "ms_per_step": 1e3 * el,
+            "stages_ms": {"transpose": float(np.mean([p["unpack_ms"] for p in prof])),
+                          "forward_fft": float(np.mean([p["fft_ms"] for p in prof])),
+                          "contraction": c_ms,
+                          "inverse_fft": float(np.mean([p["ifft_ms"] for p in prof]))},
+            "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": maxabs,
+            "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_PEAK_T[name], "unit": "TFLOP/s",
+                         "frac": tflops / MFMA_PEAK_T[name],
+                         "note": "algorithmic flops (8 per complex MAC, no padding or compensation products)"},
+        }
+        del d_out, plan
+        torch.cuda.empty_cache()
+    out["speedup_bf16_vs_fp32"] = out["bf16"]["value"] / out["fp32"]["value"]
+    out["speedup_fp16_vs_fp32"] = out["fp16"]["value"] / out["fp32"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fdd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -188,6 +276,10 @@ def main():
         "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
         "parity": parity,
     }
+    del d_out, d_in, s
+    torch.cuda.empty_cache()
+    if not args.no_fdd:
+        line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

@@ -116,11 +116,20 @@ EXPORTED = [s[0] for s in _SIG]
 
 
 def lib() -> C.CDLL:
-    """Loads libdedisp.so (raises if it has not been built)."""
+    """Loads libdedisp.so (raises if it has not been built).
+
+    PyTorch wheels carry their own copy of the HIP runtime under the same
+    soname; when torch is installed it is imported first, so that this library
+    binds to the runtime already in the process instead of loading a second
+    one (two HIP runtimes in one process cannot share the device)."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dedisp_tests_amd.build`")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = C.CDLL(str(LIB_PATH))
         for name, res, args in _SIG:
             f = getattr(l, name)

@@ -1,0 +1,50 @@
+// Internal C-ABI of the FDD kernels (see kernels.h for conventions).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// nfft values the on-chip transform engine handles.
+int dd_fdd_fft_supported(size_t nfft);
+
+// Forward transforms of every (channel, chunk): rows are channel-major 8-bit
+// samples (row_bytes per channel), chunk i starts at sample i*chunk_step and
+// spans nfft samples; x = (u8 - 127.5) * chan_scale[c] for sample t < nvalid and
+// 0 beyond. period > 0 reads the series periodically (sample t mod period; the
+// chunks then span at most two periods). Z: [nchan][nchunk][Zp] complex,
+// M = nfft/2 points of the packed transform. ph: [nfft] e^{+2 pi i m/nfft}.
+hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step,
+                          size_t nvalid, size_t period, const float* chan_scale,
+                          const float2* ph, size_t nfft, float2* Z, size_t nchan,
+                          size_t nchunk, size_t Zp, hipStream_t stream);
+
+// Inverse transforms of every (row, chunk) of Y ([nrows][nchunk][Yp], bins
+// 0..nfft/2): writes samples t < good of each chunk, scaled by 1/nfft, to
+// out[row*out_stride + chunk*good + t] for chunk*good + t < nsamps_out.
+hipError_t dd_fdd_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Yp,
+                          const float2* ph, size_t nfft, float* out, size_t out_stride,
+                          size_t good, size_t nsamps_out, hipStream_t stream);
+
+// Phasor contraction, float32 on the vector ALUs:
+//   Y[i][ch][k] = sum_c e^{+2 pi i ((k*d[i][c]) mod nfft)/nfft} X_c[ch][k],  k = 0..nfft/2
+// with X_c formed from the packed spectra Z. delays: [ndm][nchan] int32.
+hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delays, size_t ndm,
+                                size_t nchan, size_t nchunk, const float2* ph, size_t nfft,
+                                float2* Y, size_t Yp, hipStream_t stream);
+
+// Phasor contraction on the matrix cores. prec: 0 = float32 MFMA
+// (v_mfma_f32_32x32x2_f32), 1 = bfloat16 MFMA with hi+lo split phasors and
+// spectra (three products), 2 = float16 MFMA (one product); float32
+// accumulation in all three. Same inputs and output as dd_fdd_contract_fp32.
+hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const int32_t* delays,
+                                size_t ndm, size_t nchan, size_t nchunk, const float2* ph,
+                                size_t nfft, float2* Y, size_t Yp, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,5 +1,28 @@
 // FDDGPUPlan: host side of the Fourier-domain dedispersion path.
+//
+// Reference: src/fdd/FDDGPUPlan.cpp:39-706 (whole-series transform) and
+// :725-1271 (time-segmented, overlap-save). The design here:
+//  - the 8-bit input is made resident in HBM once and transposed to
+//    channel-major rows (the reference re-uploads and re-transforms it for
+//    every outer DM batch);
+//  - time series are produced by overlap-save over chunks of nfft samples
+//    (power of two, nfft > max delay): chunk i starts at sample i*good,
+//    good = nfft - max delay rounded down to even, and keeps its first good
+//    outputs. Every chunk is processed (the reference's segmented kernel stops
+//    at bin nsamp/2+1 of the flattened chunk layout, leaving trailing chunks
+//    empty);
+//  - per (channel, chunk) forward transform, per-bin phasor contraction
+//    Y_k[dm][chunk] = sum_c P_k[dm][c] X_k[c][chunk] with exact integer phase
+//    reduction (k*d) mod nfft, per (DM, chunk) inverse transform;
+//  - the killmask zeroes channels (the reference FDD ignores it).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "fdd_kernels.h"
 #include "internal.hpp"
+#include "kernels.h"
 
 namespace dedisp {
 
@@ -7,8 +30,34 @@ struct FDDState {
     dedisp_amd_fdd_output output = DEDISP_AMD_FDD_FOURIER;
     dedisp_amd_fdd_precision precision = DEDISP_AMD_FDD_FP32;
     int segmented = 0;
-    dedisp_size nfft = 0;
+    dedisp_size nfft_req = 0;
+
+    DeviceBuffer d_in;      // staged 8-bit input (host-pointer mode)
+    DeviceBuffer d_rows;    // channel-major 8-bit rows
+    DeviceBuffer d_Z;       // [nchan][nchunk][Zp] packed spectra
+    DeviceBuffer d_Y;       // [dm batch][nchunk][Yp] dedispersed spectra
+    DeviceBuffer d_out;     // staged output (host-pointer mode)
+    DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
+    DeviceBuffer d_scale;   // [nchan] per-channel input scale
+    DeviceBuffer d_ph;      // [nfft] e^{+2 pi i m / nfft}
+
+    unsigned long version = ~0ul;
+    int32_t dmin = 0, dmax = 0;
+    size_t ph_nfft = 0;
 };
+
+namespace {
+
+// Memory the dedispersed spectra of one DM batch may take.
+constexpr size_t kYBudget = size_t(24) << 30;
+
+bool env_flag(const char* name)
+{
+    const char* s = std::getenv(name);
+    return s && std::atoi(s) != 0;
+}
+
+}  // namespace
 
 FDDGPUPlan::FDDGPUPlan(size_type nchans, float_type dt, float_type f0, float_type df, int device_idx)
     : GPUPlan(nchans, dt, f0, df, device_idx), m_state(new FDDState)
@@ -18,22 +67,196 @@ FDDGPUPlan::FDDGPUPlan(size_type nchans, float_type dt, float_type f0, float_typ
 
 FDDGPUPlan::~FDDGPUPlan() {}
 
-void FDDGPUPlan::set_output_mode(dedisp_amd_fdd_output mode) { m_state->output = mode; }
+void FDDGPUPlan::set_output_mode(dedisp_amd_fdd_output mode)
+{
+    if (mode != DEDISP_AMD_FDD_FOURIER && mode != DEDISP_AMD_FDD_TIMESERIES) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    m_state->output = mode;
+}
 dedisp_amd_fdd_output FDDGPUPlan::get_output_mode() const { return m_state->output; }
-void FDDGPUPlan::set_precision(dedisp_amd_fdd_precision p) { m_state->precision = p; }
+
+void FDDGPUPlan::set_precision(dedisp_amd_fdd_precision p)
+{
+    if (p != DEDISP_AMD_FDD_FP32 && p != DEDISP_AMD_FDD_BF16 && p != DEDISP_AMD_FDD_FP16)
+        DD_THROW(DEDISP_UNKNOWN_ERROR);
+    m_state->precision = p;
+}
 dedisp_amd_fdd_precision FDDGPUPlan::get_precision() const { return m_state->precision; }
-void FDDGPUPlan::set_segmented(int enable, dedisp_size nfft) { m_state->segmented = enable; m_state->nfft = nfft; }
+
+void FDDGPUPlan::set_segmented(int enable, dedisp_size nfft)
+{
+    if (nfft && !dd_fdd_fft_supported(nfft)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    m_state->segmented = enable;
+    m_state->nfft_req = nfft;
+}
 
 dedisp_size FDDGPUPlan::output_length(dedisp_size nsamps) const
 {
-    if (m_state->output == DEDISP_AMD_FDD_FOURIER && !m_state->segmented)
-        return round_up(nsamps + m_max_delay, 16384);
+    const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");
+    if (m_state->output == DEDISP_AMD_FDD_FOURIER && !segmented)
+        return round_up(nsamps + m_max_delay, 16384);  // src/fdd/FDDGPUPlan.cpp:91
     return nsamps - m_max_delay;
 }
 
-void FDDGPUPlan::execute(size_type, const byte_type*, size_type, byte_type*, size_type, unsigned)
+void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
+                         byte_type* out, size_type out_nbits, unsigned flags)
 {
-    DD_THROW(DEDISP_UNKNOWN_ERROR);
+    if (!in || !out) DD_THROW(DEDISP_INVALID_POINTER);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    // 8-bit input, float output (src/fdd/FDDGPUPlan.cpp:66-67)
+    if (in_nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+
+    const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");
+    const bool fourier = (m_state->output == DEDISP_AMD_FDD_FOURIER) && !segmented;
+    if (fourier) DD_THROW(DEDISP_UNKNOWN_ERROR);  // whole-series Fourier output: not built yet
+    execute_timeseries(nsamps, in, out, flags);
+}
+
+void FDDGPUPlan::execute_timeseries(size_type nsamps, const byte_type* in, byte_type* out,
+                                    unsigned flags)
+{
+    GPUContext& ctx = *m_ctx;
+    FDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    ctx.timer.reset(ctx.profiling);
+    ctx.profile = dedisp_amd_profile{};
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const size_t nchan = m_nchans, ndm = m_dm_count;
+    const size_t nsc = nsamps - m_max_delay;
+    if (nsc == 0) return;
+
+    // ---- per-plan tables: truncated delays and channel scales ----
+    if (st.version != m_tables_version) {
+        std::vector<int32_t> d(ndm * nchan);
+        int32_t dmin = 0, dmax = 0;
+        for (size_t i = 0; i < ndm; ++i)
+            for (size_t c = 0; c < nchan; ++c) {
+                int32_t v = fdd_delay(h_dm_list[i], h_delay_table[c]);
+                d[i * nchan + c] = v;
+                dmin = std::min(dmin, v);
+                dmax = std::max(dmax, v);
+            }
+        // x = (u8 - 127.5) * (float)(1.0/nchan) (src/fdd/FDDGPUPlan.cpp:465-473);
+        // killed channels contribute nothing
+        std::vector<float> scale(nchan);
+        for (size_t c = 0; c < nchan; ++c) scale[c] = h_killmask[c] ? (float)(1.0 / nchan) : 0.f;
+        st.d_delays.reserve(d.size() * 4);
+        st.d_scale.reserve(nchan * 4);
+        DD_HIP(hipMemcpyAsync(st.d_delays.ptr, d.data(), d.size() * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_scale.ptr, scale.data(), nchan * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.dmin = dmin;
+        st.dmax = dmax;
+        st.version = m_tables_version;
+    }
+    if (st.dmin < 0) DD_THROW(DEDISP_UNKNOWN_ERROR);  // negative DMs: not supported
+    const size_t dmax = (size_t)st.dmax;
+
+    // ---- chunking ----
+    size_t nfft = st.nfft_req;
+    if (!nfft) {
+        // smallest contraction volume nchunk * (nfft/2 + 1) among the supported lengths
+        size_t best = ~size_t(0);
+        for (size_t cand = 8192; cand <= 65536; cand *= 2) {
+            if (cand < dmax + 2 + 2) continue;
+            const size_t good = (cand - dmax) & ~size_t(1);
+            const size_t cost = div_up(nsc, good) * (cand / 2 + 1);
+            if (cost < best) { best = cost; nfft = cand; }
+        }
+        if (!nfft) DD_THROW(DEDISP_UNKNOWN_ERROR);  // max delay beyond the largest segment
+    }
+    if (nfft < dmax + 4) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    const size_t good = (nfft - dmax) & ~size_t(1);
+    const size_t nchunk = div_up(nsc, good);
+    const size_t M = nfft / 2;
+    const size_t Zp = M, Yp = M + 64;
+
+    if (st.ph_nfft != nfft) {
+        std::vector<float> ph(2 * nfft);
+        for (size_t m = 0; m < nfft; ++m) {
+            const double a = 2.0 * M_PI * (double)m / (double)nfft;
+            ph[2 * m] = (float)std::cos(a);
+            ph[2 * m + 1] = (float)std::sin(a);
+        }
+        st.d_ph.reserve(nfft * 8);
+        DD_HIP(hipMemcpyAsync(st.d_ph.ptr, ph.data(), nfft * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.ph_nfft = nfft;
+    }
+    const float2* ph = st.d_ph.as<float2>();
+
+    // ---- input ----
+    const uint8_t* d_in;
+    if (device_ptrs) {
+        d_in = in;
+    } else {
+        st.d_in.reserve(nsamps * nchan + 64);
+        ctx.timer.begin(StageTimer::H2D, stream);
+        DD_HIP(hipMemcpyAsync(st.d_in.ptr, in, nsamps * nchan, hipMemcpyHostToDevice, stream));
+        ctx.timer.end(StageTimer::H2D, stream);
+        d_in = st.d_in.as<uint8_t>();
+    }
+
+    // ---- transpose to channel-major rows, forward transforms ----
+    const size_t row_words = div_up(std::max(nsamps, (nchunk - 1) * good + nfft), 4 * 64) * 64;
+    st.d_rows.reserve(nchan * row_words * 4);
+    st.d_Z.reserve(nchan * nchunk * Zp * 8);
+    ctx.timer.begin(StageTimer::UNPACK, stream);
+    DD_HIP(dd_tdd_transpose(d_in, nsamps, nchan, 8, nchan, st.d_rows.as<uint32_t>(), row_words, stream));
+    ctx.timer.end(StageTimer::UNPACK, stream);
+    ctx.timer.begin(StageTimer::FFT, stream);
+    DD_HIP(dd_fdd_forward(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, 0,
+                          st.d_scale.as<float>(), ph, nfft, st.d_Z.as<float2>(), nchan, nchunk, Zp,
+                          stream));
+    ctx.timer.end(StageTimer::FFT, stream);
+
+    // ---- per DM batch: contraction, inverse transforms ----
+    const size_t per_dm = nchunk * Yp * 8;
+    const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    st.d_Y.reserve(dm_batch * per_dm);
+    float* d_out;
+    if (device_ptrs) {
+        d_out = reinterpret_cast<float*>(out);
+    } else {
+        st.d_out.reserve(ndm * nsc * 4);
+        d_out = st.d_out.as<float>();
+    }
+    // DEDISP_FDD_VALU=1 selects the float32 vector-ALU contraction instead of
+    // the matrix cores (same arithmetic as the float32 MFMA form)
+    const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
+    for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
+        const size_t nb = std::min(dm_batch, ndm - b0);
+        ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+        const int32_t* dl = st.d_delays.as<int32_t>() + b0 * nchan;
+        if (valu)
+            DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Zp, dl, nb, nchan, nchunk, ph, nfft,
+                                        st.d_Y.as<float2>(), Yp, stream));
+        else
+            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Zp, dl, nb, nchan,
+                                        nchunk, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
+        ctx.timer.end(StageTimer::DEDISPERSE, stream);
+        ctx.timer.begin(StageTimer::IFFT, stream);
+        DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, d_out + b0 * nsc, nsc,
+                              good, nsc, stream));
+        ctx.timer.end(StageTimer::IFFT, stream);
+    }
+
+    if (!device_ptrs) {
+        ctx.timer.begin(StageTimer::D2H, stream);
+        DD_HIP(hipMemcpyAsync(out, d_out, ndm * nsc * 4, hipMemcpyDeviceToHost, stream));
+        ctx.timer.end(StageTimer::D2H, stream);
+    }
+    if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
+        DD_HIP(hipStreamSynchronize(stream));
+        ctx.timer.collect(ctx.profile);
+        ctx.profile.work_units = (double)ndm * (double)nchan * (double)nchunk * (double)(M + 1);
+    }
 }
 
 }  // namespace dedisp

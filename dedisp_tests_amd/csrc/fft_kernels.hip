@@ -1,0 +1,304 @@
+// gfx950 FFT kernels of the Fourier-domain dedispersion (FDD) path.
+//
+// One workgroup computes one complex transform of M = A*B*C points entirely
+// on chip: three passes of small in-register DFTs (sizes C, B, A), with the
+// inter-pass exchanges staged through LDS (in P rounds when the data exceeds
+// the LDS budget). Input and output are in natural order and every global
+// access is unit-stride across the workgroup, so a transform costs one HBM
+// read and one HBM write.
+//
+//   n = n0 + A*n1 + A*B*n2,   k = k2 + C*k1 + C*B*k0
+//   pass 1: DFT_C over n2 for each (n0,n1), twiddle W_M^{(n0+A n1) k2}
+//   pass 2: DFT_B over n1 for each (n0,k2), twiddle W_{AB}^{n0 k1}
+//   pass 3: DFT_A over n0 for each (k1,k2)
+//
+// Real transforms of length nfft = 2M use the half-length packing
+// z[m] = x[2m] + i x[2m+1]; the forward kernel writes Z (M points) and the
+// consumer forms X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]),
+// W = e^{-2 pi i/nfft}. The inverse kernel forms its packed input from the
+// M+1 bins of each DM row at load time.
+//
+// Twiddles come from one table ph[m] = e^{+2 pi i m / nfft}, m < nfft, built in
+// double precision on the host: W_M^j = conj(ph[2j]).
+#include <hip/hip_runtime.h>
+
+#include "fdd_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// In-register DFT of size R (power of two), natural order in and out,
+// Stockham radix-2; DIR = -1 forward, +1 inverse. The twiddles are compile
+// time constants after unrolling.
+template <int R, int DIR>
+__device__ __forceinline__ void dft_reg(float2 (&v)[R])
+{
+    float2 t[R];
+#pragma unroll
+    for (int ns = 1; ns < R; ns *= 2) {
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) {
+            const int jm = j % ns;
+            const double ang = DIR * M_PI * (double)jm / (double)ns;
+            const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+            float2 a = v[j];
+            float2 b = (jm == 0) ? v[j + R / 2] : cmul(v[j + R / 2], w);
+            const int d = (j / ns) * ns * 2 + jm;
+            t[d] = cadd(a, b);
+            t[d + ns] = csub(a, b);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = t[j];
+    }
+}
+
+template <int A, int B, int C>
+struct FftShape {
+    static constexpr int M = A * B * C;
+    static constexpr int T1 = A * B;  // pass-1 threads (each C values)
+    static constexpr int T2 = A * C;  // pass-2 threads (each B values)
+    static constexpr int T3 = B * C;  // pass-3 threads (each A values)
+    static constexpr int T = (T1 > T2 ? (T1 > T3 ? T1 : T3) : (T2 > T3 ? T2 : T3));
+    // LDS floats of one exchange round (one component of every value, padded rows)
+    static constexpr int X1 = A * C * (B + 1);
+    static constexpr int X2 = B * C * (A + 1);
+    static constexpr int LDS_FLOATS = X1 > X2 ? X1 : X2;
+};
+
+// Twiddle w^j for j = 0..R-1 given w = base: reseeded from the table every 8
+// steps (tw_at(j) returns the exact table value for j multiple of 8), so each
+// value is at most 7 complex products away from a correctly rounded one.
+template <int DIR>
+__device__ __forceinline__ float2 table_tw(const float2* __restrict__ ph, uint32_t idx, uint32_t mask)
+{
+    float2 tw = ph[idx & mask];
+    if (DIR < 0) tw = conjf2(tw);
+    return tw;
+}
+
+// The transform engine. load(n) gives input point n (pass-1 threads call it
+// for n = t + A*B*n2); store(k, value) receives output point k. ph is the
+// twiddle table of e^{+2 pi i m / (2M)}. Each exchange moves the real parts
+// of all values through LDS, then the imaginary parts, so at most half of a
+// thread's values are in flight (and the LDS holds one component of the
+// whole transform).
+template <int A, int B, int C, int DIR, class Load, class Store>
+__device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, Load load, Store store)
+{
+    using S = FftShape<A, B, C>;
+    constexpr int M = S::M;
+    constexpr uint32_t MASK = 2 * M - 1;
+    const int t = threadIdx.x;
+
+    // pass 1: DFT over n2, then twiddle W_M^{t k2} (recurrence reseeded every 8)
+    float2 v[C];
+    const bool act1 = t < S::T1;
+    if (act1) {
+#pragma unroll
+        for (int n2 = 0; n2 < C; ++n2) v[n2] = load(t + S::T1 * n2);
+        dft_reg<C, DIR>(v);
+        const float2 w1 = table_tw<DIR>(ph, 2u * (uint32_t)t, MASK);
+        float2 tw = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int k2 = 1; k2 < C; ++k2) {
+            if (k2 % 8 == 0) tw = table_tw<DIR>(ph, 2u * (uint32_t)t * (uint32_t)k2, MASK);
+            else tw = cmul(tw, w1);
+            v[k2] = cmul(v[k2], tw);
+        }
+    }
+    const int n0_1 = t % A, n1_1 = t / A;
+
+    // exchange 1 -> 2: element (n0, n1, k2) at (k2*A + n0)*(B+1) + n1
+    float2 u[B];
+    const int n0_2 = t % A, k2_2 = t / A;  // pass-2 thread (n0, k2)
+    const bool act2 = t < S::T2;
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        __syncthreads();
+        if (act1) {
+#pragma unroll
+            for (int k2 = 0; k2 < C; ++k2)
+                lds[(k2 * A + n0_1) * (B + 1) + n1_1] = comp ? v[k2].y : v[k2].x;
+        }
+        __syncthreads();
+        if (act2) {
+#pragma unroll
+            for (int n1 = 0; n1 < B; ++n1) {
+                const float val = lds[(k2_2 * A + n0_2) * (B + 1) + n1];
+                if (comp) u[n1].y = val;
+                else u[n1].x = val;
+            }
+        }
+    }
+    // pass 2: DFT over n1, then twiddle W_M^{C n0 k1}
+    if (act2) {
+        dft_reg<B, DIR>(u);
+        const float2 w2 = table_tw<DIR>(ph, 2u * (uint32_t)(C * n0_2), MASK);
+        float2 tw = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int k1 = 1; k1 < B; ++k1) {
+            if (k1 % 8 == 0) tw = table_tw<DIR>(ph, 2u * (uint32_t)(C * n0_2) * (uint32_t)k1, MASK);
+            else tw = cmul(tw, w2);
+            u[k1] = cmul(u[k1], tw);
+        }
+    }
+
+    // exchange 2 -> 3: element (n0, k1, k2) at (k2*B + k1)*(A+1) + n0
+    float2 q[A];
+    const int k2_3 = t % C, k1_3 = t / C;
+    const bool act3 = t < S::T3;
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        __syncthreads();
+        if (act2) {
+#pragma unroll
+            for (int k1 = 0; k1 < B; ++k1)
+                lds[(k2_2 * B + k1) * (A + 1) + n0_2] = comp ? u[k1].y : u[k1].x;
+        }
+        __syncthreads();
+        if (act3) {
+#pragma unroll
+            for (int n0 = 0; n0 < A; ++n0) {
+                const float val = lds[(k2_3 * B + k1_3) * (A + 1) + n0];
+                if (comp) q[n0].y = val;
+                else q[n0].x = val;
+            }
+        }
+    }
+    // pass 3: DFT over n0; output k = k2 + C k1 + C B k0 = t + T3 k0
+    if (act3) {
+        dft_reg<A, DIR>(q);
+#pragma unroll
+        for (int k0 = 0; k0 < A; ++k0) store(t + S::T3 * k0, q[k0]);
+    }
+}
+
+// Forward: channel c, chunk i of the channel-major 8-bit rows; writes Z.
+template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
+__global__ __launch_bounds__(NT) void fdd_forward_k(
+    const uint8_t* __restrict__ rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
+    size_t period, const float* __restrict__ chan_scale, const float2* __restrict__ ph,
+    float2* __restrict__ Z, size_t nchunk, size_t Zp)
+{
+    using S = FftShape<A, B, C>;
+    __shared__ float lds[S::LDS_FLOATS];
+    const size_t chunk = blockIdx.x, c = blockIdx.y;
+    const uint8_t* src = rows + c * row_bytes;
+    const size_t t0 = chunk * chunk_step;
+    const float s = chan_scale[c];
+    float2* dst = Z + (c * nchunk + chunk) * Zp;
+    // x[t] = (u8 - 127.5) * scale (src/fdd/unpack/unpack_kernel.cuh:60) for
+    // t < nvalid, zero beyond (the zero padding of src/fdd/FDDGPUPlan.cpp:476-483);
+    // with period > 0 the series is read periodically (t mod period).
+    auto sample = [&](size_t t) -> float {
+        if (period) t = (t >= period) ? t - period : t;
+        return (t < nvalid) ? ((float)src[t] - 127.5f) * s : 0.f;
+    };
+    auto load = [&](int n) -> float2 {
+        const size_t t = t0 + 2 * (size_t)n;
+        if (!period && t + 1 < nvalid) {
+            const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t);
+            return make_float2(((float)(pair & 0xff) - 127.5f) * s, ((float)(pair >> 8) - 127.5f) * s);
+        }
+        return make_float2(sample(t), sample(t + 1));
+    };
+    auto store = [&](int k, float2 val) { dst[k] = val; };
+    fft3<A, B, C, -1>(ph, lds, load, store);
+}
+
+// Inverse: DM row j, chunk i of Y (M+1 bins); writes the first `good`
+// samples of the chunk's real series, times 1/nfft, into out.
+template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
+__global__ __launch_bounds__(NT) void fdd_inverse_k(
+    const float2* __restrict__ Y, size_t nchunk, size_t Yp, const float2* __restrict__ ph,
+    float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out)
+{
+    using S = FftShape<A, B, C>;
+    constexpr int M = S::M;
+    __shared__ float lds[S::LDS_FLOATS];
+    const size_t chunk = blockIdx.x, row = blockIdx.y;
+    const float2* src = Y + (row * nchunk + chunk) * Yp;
+    float* dst = out + row * out_stride;
+    const size_t t0 = chunk * good;
+    const float inv_n = 1.0f / (float)(2 * M);
+    auto load = [&](int k) -> float2 {
+        // Zu[k] = (Y[k] + conj Y[M-k]) + i e^{+2 pi i k/nfft} (Y[k] - conj Y[M-k])
+        float2 a = src[k];
+        float2 b = conjf2(src[M - k]);
+        if (k == 0) a.y = 0.f, b = make_float2(src[M].x, 0.f);
+        float2 e = cadd(a, b), d = csub(a, b);
+        float2 o = cmul(d, ph[k]);
+        return make_float2(e.x - o.y, e.y + o.x);
+    };
+    auto store = [&](int m, float2 val) {
+        const size_t ta = 2 * (size_t)m, tb = ta + 1;
+        if (ta < good && t0 + ta < nsamps_out) dst[t0 + ta] = val.x * inv_n;
+        if (tb < good && t0 + tb < nsamps_out) dst[t0 + tb] = val.y * inv_n;
+    };
+    fft3<A, B, C, +1>(ph, lds, load, store);
+}
+
+template <int A, int B, int C>
+hipError_t launch_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
+                          size_t period, const float* chan_scale, const float2* ph, float2* Z,
+                          size_t nchan, size_t nchunk, size_t Zp, hipStream_t stream)
+{
+    hipLaunchKernelGGL((fdd_forward_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nchan),
+                       dim3(FftShape<A, B, C>::T), 0, stream, rows, row_bytes, chunk_step, nvalid,
+                       period, chan_scale, ph, Z, nchunk, Zp);
+    return hipGetLastError();
+}
+
+template <int A, int B, int C>
+hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
+                          float* out, size_t out_stride, size_t good, size_t nsamps_out,
+                          hipStream_t stream)
+{
+    hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nrows),
+                       dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
+                       good, nsamps_out);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int dd_fdd_fft_supported(size_t nfft)
+{
+    return nfft == 8192 || nfft == 16384 || nfft == 32768 || nfft == 65536;
+}
+
+extern "C" hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step,
+                                     size_t nvalid, size_t period, const float* chan_scale,
+                                     const float2* ph, size_t nfft, float2* Z, size_t nchan,
+                                     size_t nchunk, size_t Zp, hipStream_t stream)
+{
+#define DD_FWD(A, B, C) launch_forward<A, B, C>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, Z, nchan, nchunk, Zp, stream)
+    switch (nfft) {
+    case 8192:  return DD_FWD(16, 16, 16);
+    case 16384: return DD_FWD(16, 16, 32);
+    case 32768: return DD_FWD(16, 32, 32);
+    case 65536: return DD_FWD(32, 32, 32);
+#undef DD_FWD
+    default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t dd_fdd_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Yp,
+                                     const float2* ph, size_t nfft, float* out, size_t out_stride,
+                                     size_t good, size_t nsamps_out, hipStream_t stream)
+{
+    switch (nfft) {
+    case 8192:  return launch_inverse<16, 16, 16>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream);
+    case 16384: return launch_inverse<16, 16, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream);
+    case 32768: return launch_inverse<16, 32, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream);
+    case 65536: return launch_inverse<32, 32, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream);
+    default: return hipErrorInvalidValue;
+    }
+}

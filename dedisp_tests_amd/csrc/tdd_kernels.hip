@@ -441,7 +441,8 @@ extern "C" hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row
 #define DD_LDS_TILINGS(X) \
     X(1, 1, 2, 4) X(1, 1, 4, 4) \
     X(2, 1, 2, 4) X(2, 2, 2, 4) X(2, 1, 4, 4) \
-    X(4, 2, 2, 4) X(4, 2, 4, 4) X(4, 1, 4, 4) X(4, 2, 2, 8) \
+    X(4, 2, 2, 4) X(4, 2, 4, 4) X(4, 1, 4, 4) X(4, 2, 2, 8) X(4, 4, 2, 4) X(4, 2, 4, 8) \
+    X(4, 2, 1, 4) X(4, 1, 2, 8) \
     X(8, 2, 2, 4) X(8, 4, 2, 4) X(8, 2, 4, 4) X(8, 2, 2, 8)
 
 extern "C" int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tl)

@@ -1,0 +1,176 @@
+"""FDD parity on the GPU against the FP64 oracle.
+
+Tolerance (the FDD-FP32 bar): relative L2 error <= 1e-4 against the float64
+shift-sum; the reference's own float32 phase formula stays about 1e-3 from it,
+so this is a tighter bar than bit-equality with the reference would give.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLD = Path(__file__).resolve().parent / "golden"
+f32 = np.float32
+pytestmark = pytest.mark.gpu
+
+REL_FP32 = 1e-4
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _fdd_plan(dd, nch, dt, f0, df, nfft=0, prec=None):
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    if nfft:
+        plan.fdd_set_segmented(True, nfft)
+    if prec is not None:
+        plan.fdd_set_precision(prec)
+    return plan
+
+
+def test_fdd_golden_timeseries(dd):
+    g = np.load(GOLD / "fdd_small.npz")
+    nch, dt, f0, df = g["geometry"]
+    plan = _fdd_plan(dd, int(nch), dt, f0, df)
+    plan.set_dm_list(g["dms"])
+    assert plan.max_delay == int(g["max_delay"][0])
+    out = plan.execute(g["input"], 8, 32)
+    assert out.shape == g["timeseries"].shape
+    assert rel_l2(out, g["timeseries"]) < REL_FP32
+
+
+@pytest.mark.parametrize("nfft", [8192, 16384, 32768, 65536])
+def test_fdd_segment_lengths_vs_oracle(dd, orc, nfft):
+    rng = np.random.default_rng(nfft)
+    nch, nsamps = 128, 70000 + 123
+    dt, f0, df = 64e-6, 1581.8, -0.78125
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 450, 21, dtype=f32)
+    plan = _fdd_plan(dd, nch, dt, f0, df, nfft=nfft)
+    plan.set_dm_list(dms)
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(out, ref) < REL_FP32
+
+
+def test_fdd_cfg1_timeseries_and_pulse(dd, orc):
+    from dedisp_tests_amd import synth
+    nch, dt, f0, df = 256, 64e-6, 1581.8, -0.390625
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    s = synth.filterbank(65536, nch, 8, 128.0, 20.0, seed=1, pulses=[(100.0, 30000, 4, 30)], delays=delays)
+    data = s.astype(np.uint8)
+    dms = np.arange(256, dtype=f32)
+    plan = _fdd_plan(dd, nch, dt, f0, df)
+    plan.set_dm_list(dms)
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_timeseries(nch, delays, dms, 65536, plan.max_delay, data)
+    assert rel_l2(out, ref) < REL_FP32
+    # per-DM max error well below the noise
+    assert np.max(np.abs(out - ref)) < 1e-3
+    # injected pulse: S/N at DM 100 within 0.5 % of the oracle's
+    def snr(x):
+        return (x.max() - np.median(x)) / x.std()
+    assert abs(snr(out[100]) / snr(ref[100]) - 1) < 5e-3
+    assert int(np.argmax(out.max(axis=1))) in (99, 100, 101)
+
+
+def test_fdd_use_segmented_env_and_killmask(dd, orc, monkeypatch):
+    rng = np.random.default_rng(4)
+    nch, nsamps = 64, 30000
+    dt, f0, df = 64e-6, 1581.8, -2.0
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 60, 11, dtype=f32)
+    monkeypatch.setenv("USE_SEGMENTED", "1")
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)      # default output, segmented by env
+    plan.set_dm_list(dms)
+    assert plan.out_length(nsamps) == nsamps - plan.max_delay
+    km = np.ones(nch, np.int32)
+    km[[3, 17, 40]] = 0
+    plan.set_killmask(km)
+    out = plan.execute(data, 8, 32)
+    keep = np.nonzero(km)[0]
+    ref = orc.fdd_timeseries(len(keep), delays[keep], dms, nsamps, plan.max_delay,
+                             np.ascontiguousarray(data[:, keep])) * (len(keep) / nch)
+    assert rel_l2(out, ref) < REL_FP32
+
+
+def test_fdd_device_pointers(dd, orc):
+    import torch
+    rng = np.random.default_rng(8)
+    nch, nsamps = 128, 50000
+    dt, f0, df = 64e-6, 1581.8, -0.78125
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 300, 40, dtype=f32)
+    plan = _fdd_plan(dd, nch, dt, f0, df)
+    plan.set_dm_list(dms)
+    nsc = nsamps - plan.max_delay
+    d_in = torch.from_numpy(data).cuda()
+    d_out = torch.zeros((dms.size, nsc), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+    ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(d_out.cpu().numpy(), ref) < REL_FP32
+
+
+def test_fdd_validation(dd):
+    plan = _fdd_plan(dd, 64, 64e-6, 1581.8, -2.0)
+    buf = np.zeros((1000, 64), np.uint8)
+    out = np.zeros(100000, np.float32)
+    assert plan.execute_raw(1000, buf, 8, out, 32) == dd.DEDISP_NO_DM_LIST_SET
+    plan.set_dm_list(np.array([0.0, 5.0], f32))
+    assert plan.execute_raw(1000, buf, 4, out, 32) == dd.DEDISP_UNSUPPORTED_IN_NBITS
+    assert plan.execute_raw(1000, buf, 8, out, 16) == dd.DEDISP_UNSUPPORTED_OUT_NBITS
+    assert plan.execute_raw(1000, None, 8, out, 32) == dd.DEDISP_INVALID_POINTER
+    # FDD plans have no gulp / adv / guru interface (src/clib/dedisp.cpp:182, 428, 488)
+    assert dd.lib().dedisp_set_gulp_size(plan.handle, 10) == dd.DEDISP_INVALID_PLAN
+    assert plan.execute_adv(1000, buf, 8, 64, out, 32, 4000) == dd.DEDISP_INVALID_PLAN
+
+
+# Tolerances of the contraction forms against the FP64 oracle (relative L2):
+# float32 MFMA and the three-product bfloat16 form carry float32-level error;
+# the single-product float16 form carries the float16 rounding of phasors and
+# spectra (about 2^-11 per term).
+PREC_TOL = {"fp32": 1e-4, "bf16": 1e-4, "fp16": 1e-3}
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_fdd_contraction_forms_vs_oracle(dd, orc, prec):
+    from dedisp_tests_amd import synth
+    p = {"fp32": dd.FDD_FP32, "bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec]
+    nch, dt, f0, df = 256, 64e-6, 1581.8, -0.390625
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    s = synth.filterbank(65536, nch, 8, 128.0, 20.0, seed=7, pulses=[(60.0, 20000, 4, 30)], delays=delays)
+    data = s.astype(np.uint8)
+    dms = np.arange(0, 300, 1.1, dtype=f32)     # 273 DMs: a partial DM block
+    plan = _fdd_plan(dd, nch, dt, f0, df, prec=p)
+    plan.set_dm_list(dms)
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_timeseries(nch, delays, dms, 65536, plan.max_delay, data)
+    err = rel_l2(out, ref)
+    assert err < PREC_TOL[prec], err
+    i = int(np.argmin(np.abs(dms - 60.0)))
+    snr = lambda x: (x.max() - np.median(x)) / x.std()
+    assert abs(snr(out[i]) / snr(ref[i]) - 1) < 5e-3
+
+
+def test_fdd_valu_and_mfma_fp32_agree(dd, orc, monkeypatch):
+    rng = np.random.default_rng(12)
+    nch, nsamps = 96, 40000
+    dt, f0, df = 64e-6, 1581.8, -1.0
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    dms = np.linspace(0, 100, 37, dtype=f32)
+    outs = []
+    for valu in ("1", "0"):
+        monkeypatch.setenv("DEDISP_FDD_VALU", valu)
+        plan = _fdd_plan(dd, nch, dt, f0, df)
+        plan.set_dm_list(dms)
+        outs.append(plan.execute(data, 8, 32))
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(outs[0], ref) < REL_FP32
+    assert rel_l2(outs[1], ref) < REL_FP32
