@@ -30,6 +30,15 @@ hipError_t dd_fdd_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
                           const float2* ph, size_t nfft, float* out, size_t out_stride,
                           size_t good, size_t nsamps_out, hipStream_t stream);
 
+// Real transforms of length N = 16384*m of nrows rows of S (float, row
+// stride s_stride >= N, even), bins 0..N/2-1 written as complex to out (row
+// stride out_stride complex). ph16384: e^{+2 pi i j/16384}; twL: e^{-2 pi i j/L},
+// twN: e^{-2 pi i k/N} for j, k < L = N/2. work: nrows*L complex. S is
+// overwritten.
+hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t nrows, size_t N,
+                            const float2* ph16384, const float2* twL, const float2* twN,
+                            float2* work, float2* out, size_t out_stride, hipStream_t stream);
+
 // Phasor contraction, float32 on the vector ALUs:
 //   Y[i][ch][k] = sum_c e^{+2 pi i ((k*d[i][c]) mod nfft)/nfft} X_c[ch][k],  k = 0..nfft/2
 // with X_c formed from the packed spectra Z. delays: [ndm][nchan] int32.

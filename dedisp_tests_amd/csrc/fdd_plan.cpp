@@ -41,6 +41,14 @@ struct FDDState {
     DeviceBuffer d_scale;   // [nchan] per-channel input scale
     DeviceBuffer d_ph;      // [nfft] e^{+2 pi i m / nfft}
 
+    // whole-series Fourier output: row transforms of length N
+    DeviceBuffer d_S;       // [DM batch][N] circular shift-sums, then packed spectra
+    DeviceBuffer d_work;    // [DM batch][N/2] complex
+    DeviceBuffer d_ph16k;   // [16384] e^{+2 pi i j/16384}
+    DeviceBuffer d_twL;     // [N/2] e^{-2 pi i j/(N/2)}
+    DeviceBuffer d_twN;     // [N/2] e^{-2 pi i k/N}
+    size_t tw_N = 0;
+
     unsigned long version = ~0ul;
     int32_t dmin = 0, dmax = 0;
     size_t ph_nfft = 0;
@@ -113,12 +121,18 @@ void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbi
 
     const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");
     const bool fourier = (m_state->output == DEDISP_AMD_FDD_FOURIER) && !segmented;
-    if (fourier) DD_THROW(DEDISP_UNKNOWN_ERROR);  // whole-series Fourier output: not built yet
-    execute_timeseries(nsamps, in, out, flags);
+    execute_fdd(nsamps, in, out, flags, fourier);
 }
 
-void FDDGPUPlan::execute_timeseries(size_type nsamps, const byte_type* in, byte_type* out,
-                                    unsigned flags)
+// Time-series output: overlap-save over the input, outputs t < nsamps - max_delay.
+// Fourier output (src/fdd/FDDGPUPlan.cpp:85-93, 394-407, 618-650): with
+// N = round_up(nsamps + max_delay, 16384), sum_c X_c[k] e^{2 pi i k d_c/N} is the
+// N-point transform of the circular shift-sum s[t] = sum_c x_c[(t + d_c) mod N]
+// (x zero beyond nsamps). s is produced by the same overlap-save path over the
+// N-periodic extension of the input (t = 0 .. N-1), then transformed per DM;
+// bins 0..N/2-1 are written as N floats (re, im).
+void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* out,
+                             unsigned flags, bool fourier)
 {
     GPUContext& ctx = *m_ctx;
     FDDState& st = *m_state;
@@ -128,8 +142,11 @@ void FDDGPUPlan::execute_timeseries(size_type nsamps, const byte_type* in, byte_
     ctx.profile = dedisp_amd_profile{};
     const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
     const size_t nchan = m_nchans, ndm = m_dm_count;
-    const size_t nsc = nsamps - m_max_delay;
+    const size_t Nf = round_up(nsamps + m_max_delay, 16384);   // whole-series length
+    const size_t nsc = fourier ? Nf : nsamps - m_max_delay;    // outputs per DM (t)
+    const size_t out_floats = fourier ? Nf : nsc;              // floats per output row
     if (nsc == 0) return;
+    if (fourier && Nf >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
 
     // ---- per-plan tables: truncated delays and channel scales ----
     if (st.version != m_tables_version) {
@@ -203,28 +220,59 @@ void FDDGPUPlan::execute_timeseries(size_type nsamps, const byte_type* in, byte_
         d_in = st.d_in.as<uint8_t>();
     }
 
+    if (fourier && st.tw_N != Nf) {
+        const size_t L = Nf / 2;
+        std::vector<float> a(2 * L), b(2 * L), c(2 * 16384);
+        for (size_t j = 0; j < L; ++j) {
+            const double t1 = -2.0 * M_PI * (double)j / (double)L;
+            const double t2 = -2.0 * M_PI * (double)j / (double)Nf;
+            a[2 * j] = (float)std::cos(t1);
+            a[2 * j + 1] = (float)std::sin(t1);
+            b[2 * j] = (float)std::cos(t2);
+            b[2 * j + 1] = (float)std::sin(t2);
+        }
+        for (size_t j = 0; j < 16384; ++j) {
+            const double t = 2.0 * M_PI * (double)j / 16384.0;
+            c[2 * j] = (float)std::cos(t);
+            c[2 * j + 1] = (float)std::sin(t);
+        }
+        st.d_twL.reserve(L * 8);
+        st.d_twN.reserve(L * 8);
+        st.d_ph16k.reserve(16384 * 8);
+        DD_HIP(hipMemcpyAsync(st.d_twL.ptr, a.data(), L * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_twN.ptr, b.data(), L * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_ph16k.ptr, c.data(), 16384 * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.tw_N = Nf;
+    }
+
     // ---- transpose to channel-major rows, forward transforms ----
-    const size_t row_words = div_up(std::max(nsamps, (nchunk - 1) * good + nfft), 4 * 64) * 64;
+    const size_t in_span = fourier ? nsamps : std::max(nsamps, (nchunk - 1) * good + nfft);
+    const size_t row_words = div_up(in_span, 4 * 64) * 64;
     st.d_rows.reserve(nchan * row_words * 4);
     st.d_Z.reserve(nchan * nchunk * Zp * 8);
     ctx.timer.begin(StageTimer::UNPACK, stream);
     DD_HIP(dd_tdd_transpose(d_in, nsamps, nchan, 8, nchan, st.d_rows.as<uint32_t>(), row_words, stream));
     ctx.timer.end(StageTimer::UNPACK, stream);
     ctx.timer.begin(StageTimer::FFT, stream);
-    DD_HIP(dd_fdd_forward(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, 0,
+    DD_HIP(dd_fdd_forward(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, fourier ? Nf : 0,
                           st.d_scale.as<float>(), ph, nfft, st.d_Z.as<float2>(), nchan, nchunk, Zp,
                           stream));
     ctx.timer.end(StageTimer::FFT, stream);
 
     // ---- per DM batch: contraction, inverse transforms ----
-    const size_t per_dm = nchunk * Yp * 8;
+    const size_t per_dm = nchunk * Yp * 8 + (fourier ? 2 * Nf * 4 : 0);
     const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
-    st.d_Y.reserve(dm_batch * per_dm);
+    st.d_Y.reserve(dm_batch * nchunk * Yp * 8);
+    if (fourier) {
+        st.d_S.reserve(dm_batch * Nf * 4);
+        st.d_work.reserve(dm_batch * Nf * 4);
+    }
     float* d_out;
     if (device_ptrs) {
         d_out = reinterpret_cast<float*>(out);
     } else {
-        st.d_out.reserve(ndm * nsc * 4);
+        st.d_out.reserve(ndm * out_floats * 4);
         d_out = st.d_out.as<float>();
     }
     // DEDISP_FDD_VALU=1 selects the float32 vector-ALU contraction instead of
@@ -242,14 +290,23 @@ void FDDGPUPlan::execute_timeseries(size_type nsamps, const byte_type* in, byte_
                                         nchunk, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
         ctx.timer.begin(StageTimer::IFFT, stream);
-        DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, d_out + b0 * nsc, nsc,
-                              good, nsc, stream));
+        if (!fourier) {
+            DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, d_out + b0 * nsc, nsc,
+                                  good, nsc, stream));
+        } else {
+            DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, st.d_S.as<float>(), Nf,
+                                  good, Nf, stream));
+            DD_HIP(dd_fdd_rfft_rows(st.d_S.as<float>(), Nf, nb, Nf, st.d_ph16k.as<float2>(),
+                                    st.d_twL.as<float2>(), st.d_twN.as<float2>(),
+                                    st.d_work.as<float2>(),
+                                    reinterpret_cast<float2*>(d_out + b0 * out_floats), Nf / 2, stream));
+        }
         ctx.timer.end(StageTimer::IFFT, stream);
     }
 
     if (!device_ptrs) {
         ctx.timer.begin(StageTimer::D2H, stream);
-        DD_HIP(hipMemcpyAsync(out, d_out, ndm * nsc * 4, hipMemcpyDeviceToHost, stream));
+        DD_HIP(hipMemcpyAsync(out, d_out, ndm * out_floats * 4, hipMemcpyDeviceToHost, stream));
         ctx.timer.end(StageTimer::D2H, stream);
     }
     if (!(device_ptrs && (flags & DEDISP_ASYNC))) {

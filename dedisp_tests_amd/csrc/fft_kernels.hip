@@ -198,7 +198,7 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     // t < nvalid, zero beyond (the zero padding of src/fdd/FDDGPUPlan.cpp:476-483);
     // with period > 0 the series is read periodically (t mod period).
     auto sample = [&](size_t t) -> float {
-        if (period) t = (t >= period) ? t - period : t;
+        if (period) t %= period;
         return (t < nvalid) ? ((float)src[t] - 127.5f) * s : 0.f;
     };
     auto load = [&](int n) -> float2 {
@@ -267,7 +267,100 @@ hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Real transform of length N = 16384*m per row (whole-series Fourier output).
+// Packed z[n] = s[2n] + i s[2n+1], L = N/2 = 8192*m points, four-step with
+// n = n1 + m*n2, k = k2 + 8192*k1:
+//   step A: 8192-point DFT over n2 for each (row, n1), times W_L^{n1 k2}
+//   step B: m-point DFT over n1 for each (row, k2)
+//   step C: real-input pairing X[k] = (Z[k] + conj Z[L-k])/2
+//                                     - i/2 W_N^k (Z[k] - conj Z[L-k])
+// twL[j] = e^{-2 pi i j/L}, twN[k] = e^{-2 pi i k/N} (j, k < L), built in double.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(512) void fourier_step_a_k(
+    const float* __restrict__ S, size_t s_stride, size_t m, const float2* __restrict__ ph8192,
+    const float2* __restrict__ twL, float2* __restrict__ A)
+{
+    using Sh = FftShape<16, 16, 32>;
+    __shared__ float lds[Sh::LDS_FLOATS];
+    const size_t n1 = blockIdx.x, row = blockIdx.y;
+    const float* src = S + row * s_stride;
+    float2* dst = A + (row * m + n1) * 8192;
+    auto load = [&](int n2) -> float2 {
+        const size_t n = n1 + m * (size_t)n2;
+        return make_float2(src[2 * n], src[2 * n + 1]);
+    };
+    auto store = [&](int k2, float2 v) { dst[k2] = cmul(v, twL[n1 * (size_t)k2]); };
+    fft3<16, 16, 32, -1>(ph8192, lds, load, store);
+}
+
+__global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict__ A, size_t m,
+                                                        const float2* __restrict__ twL,
+                                                        float2* __restrict__ Z, size_t z_stride)
+{
+    const size_t k2 = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t row = blockIdx.y;
+    if (k2 >= 8192) return;
+    const float2* a = A + row * m * 8192 + k2;
+    float2* z = Z + row * z_stride + k2;
+    for (size_t k1 = 0; k1 < m; ++k1) {
+        float2 acc = make_float2(0.f, 0.f);
+        size_t e = 0;  // (n1 * k1) mod m
+        for (size_t n1 = 0; n1 < m; ++n1) {
+            const float2 w = twL[e * 8192];
+            const float2 v = a[n1 * 8192];
+            acc.x = fmaf(v.x, w.x, fmaf(-v.y, w.y, acc.x));
+            acc.y = fmaf(v.x, w.y, fmaf(v.y, w.x, acc.y));
+            e += k1;
+            if (e >= m) e -= m;
+        }
+        z[k1 * 8192] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict__ Z, size_t z_stride,
+                                                        size_t L, const float2* __restrict__ twN,
+                                                        float2* __restrict__ out, size_t out_stride)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t row = blockIdx.y;
+    if (k >= L) return;
+    const float2* z = Z + row * z_stride;
+    const float2 a = z[k];
+    const float2 bz = z[k ? L - k : 0];
+    const float2 e = make_float2(0.5f * (a.x + bz.x), 0.5f * (a.y - bz.y));
+    const float2 d = make_float2(a.x - bz.x, a.y + bz.y);
+    const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+    const float2 r = cmul(o, twN[k]);
+    out[row * out_stride + k] = make_float2(e.x + r.x, e.y + r.y);
+}
+
 }  // namespace
+
+extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t nrows, size_t N,
+                                       const float2* ph16384, const float2* twL, const float2* twN,
+                                       float2* work, float2* out, size_t out_stride,
+                                       hipStream_t stream)
+{
+    if (N % 16384) return hipErrorInvalidValue;
+    const size_t m = N / 16384, L = N / 2;
+    // the 8192-point engine uses the 16384-point phasor table (ph[2j] = W_8192^-j)
+    hipLaunchKernelGGL(fourier_step_a_k, dim3((unsigned)m, (unsigned)nrows),
+                       dim3(512), 0, stream, S, s_stride, m, ph16384, twL, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // step B writes the packed spectrum over the rows of S (L complex = N floats per row)
+    float2* Z = reinterpret_cast<float2*>(const_cast<float*>(S));
+    const size_t z_stride = s_stride / 2;
+    hipLaunchKernelGGL(fourier_step_b_k, dim3(32, (unsigned)nrows), dim3(256), 0, stream, work, m,
+                       twL, Z, z_stride);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fourier_step_c_k, dim3((unsigned)((L + 255) / 256), (unsigned)nrows), dim3(256),
+                       0, stream, Z, z_stride, L, twN, out, out_stride);
+    return hipGetLastError();
+}
 
 extern "C" int dd_fdd_fft_supported(size_t nfft)
 {

@@ -48,15 +48,19 @@ namespace {
 dd_tdd_tiling default_tiling(unsigned nbits)
 {
     dd_tdd_tiling t{};
+    // Measured on MI355X for 1024 channels x 2^20 4-bit samples x 1167 DMs:
+    // 4 DMs per wave and 16-channel staging chunks took the channel sum from
+    // 67.9 ms (2 DMs, 64 channels) to 29.7 ms; the other widths use the same
+    // shape with the samples-per-lane their tiles are instantiated for.
     t.waves = 4;
-    t.dms_per_wave = 2;
+    t.dms_per_wave = 4;
     switch (nbits) {
     case 1: t.samples_per_lane = 32; break;
     case 2: t.samples_per_lane = 16; break;
     case 4: t.samples_per_lane = 16; break;
     default: t.samples_per_lane = 8; break;
     }
-    t.chunk_channels = 64;
+    t.chunk_channels = 16;
     return t;
 }
 

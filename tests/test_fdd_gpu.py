@@ -174,3 +174,35 @@ def test_fdd_valu_and_mfma_fp32_agree(dd, orc, monkeypatch):
     ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
     assert rel_l2(outs[0], ref) < REL_FP32
     assert rel_l2(outs[1], ref) < REL_FP32
+
+
+# ---------------------------------------------------------------------------
+# Whole-series Fourier-coefficient output (the reference fork's default)
+# ---------------------------------------------------------------------------
+
+def test_fdd_fourier_golden(dd):
+    g = np.load(GOLD / "fdd_small.npz")
+    nch, dt, f0, df = g["geometry"]
+    plan = dd.Plan(int(nch), dt, f0, df, impl=dd.FDD)      # default output mode
+    plan.set_dm_list(g["dms"])
+    nsamps = g["input"].shape[0]
+    assert plan.out_length(nsamps) == g["fourier"].shape[1]
+    out = plan.execute(g["input"], 8, 32)
+    assert rel_l2(out, g["fourier"]) < REL_FP32
+
+
+@pytest.mark.parametrize("nsamps,nch", [(40000, 64), (65536, 256), (2 ** 19, 1024)])
+def test_fdd_fourier_vs_oracle(dd, orc, nsamps, nch):
+    rng = np.random.default_rng(nsamps)
+    dt, f0, df = 54.6e-6, 1530.0, -0.3125 * 1024 / nch
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch)).astype(np.float32)), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.array([0.0, 37.3, 511.0, 1023.5], f32) if nch == 1024 else np.linspace(0, 300, 9, dtype=f32)
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.set_dm_list(dms)
+    N = plan.out_length(nsamps)
+    assert N == ((nsamps + plan.max_delay + 16383) // 16384) * 16384
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_fourier(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < REL_FP32
