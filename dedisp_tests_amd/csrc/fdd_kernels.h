@@ -49,10 +49,16 @@ hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delay
 // Phasor contraction on the matrix cores. prec: 0 = float32 MFMA
 // (v_mfma_f32_32x32x2_f32), 1 = bfloat16 MFMA with hi+lo split phasors and
 // spectra (three products), 2 = float16 MFMA (one product); float32
-// accumulation in all three. Same inputs and output as dd_fdd_contract_fp32.
-hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const int32_t* delays,
-                                size_t ndm, size_t nchan, size_t nchunk, const float2* ph,
-                                size_t nfft, float2* Y, size_t Yp, hipStream_t stream);
+// accumulation in all three. Same output as dd_fdd_contract_fp32; the delays
+// enter as phase steps q16[i][c] = (d[i][c] * 65536/nfft) mod 65536 (row pitch
+// qpitch, a multiple of 16 and >= nchan rounded up to 16, zero padded), and
+// tab65536 holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
+// (b < 256). The spectra are multiplied by xscale before the MFMA inputs are
+// formed and Y by 1/xscale after (a power of two keeping float16 in range).
+hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const uint16_t* q16,
+                                size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
+                                const float2* tab65536, const float2* ph, size_t nfft,
+                                float xscale, float2* Y, size_t Yp, hipStream_t stream);
 
 #ifdef __cplusplus
 }

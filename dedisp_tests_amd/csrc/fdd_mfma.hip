@@ -3,21 +3,34 @@
 // For every Fourier bin k the dedispersed spectra are a complex GEMM
 //   Y_k[dm][chunk] = sum_c P_k[dm][c] X_k[c][chunk],  P_k[dm][c] = e^{+2 pi i k d(dm,c)/nfft}
 // (the phase rotation of src/fdd/dedisperse/fdd_kernel.cuh:226-235 with the
-// truncated integer delay d). A wave owns a 32 (DM) x 32 (chunk) tile of NB
-// consecutive bins and walks the channels 16 at a time:
-//   A operand: phasors generated in registers — exact phase m = (k*d) mod nfft,
-//              e^{2 pi i m/nfft} = T_hi[m>>8] * T_lo[m & 255] from two 256-entry
-//              LDS tables (built in double precision on the host);
-//   B operand: X_k[c][chunk] staged in LDS for the workgroup's bins, formed on
-//              load from the packed forward transform (real-input pairing);
-//   complex MAC as four real MFMAs, Yr += Pr Xr - Pi Xi, Yi += Pr Xi + Pi Xr.
-// Forms: float32 (v_mfma_f32_32x32x2_f32, 8 MFMAs per real product and
-// K-step), bfloat16 (v_mfma_f32_32x32x16_bf16, phasors and spectra split into
-// hi + lo, three products Ph Xh + Ph Xl + Pl Xh), float16
-// (v_mfma_f32_32x32x16_f16, one product). Float32 accumulation throughout.
-// The k-positions of a K-step are assigned the same way in A and B (lane group
-// g = lane/32, element s -> channel 8g+s), so the sum runs over channels
-// whatever the hardware's k order.
+// truncated integer delay d). A workgroup owns NB consecutive bins, 32 chunks
+// and WAVES*32 DMs; a wave owns a 32 (DM) x 32 (chunk) tile of every bin and
+// walks the channels 16 at a time (one K-step).
+//
+// Phasors (A operand), generated in registers:
+//   phase in units of 2 pi/65536: q = (d * 65536/nfft) mod 65536 per (DM,
+//   channel), precomputed as uint16; the phase of bin k is k*q mod 65536
+//   (exact integer arithmetic). e^{2 pi i m/65536} = T_hi[m>>8] * T_lo[m&255]
+//   from two 256-entry LDS tables built in double precision. Per (DM,
+//   channel) and workgroup two lookups, P at the first bin and the step
+//   W = e^{2 pi i q/65536}; the following bins are P <- P*W (NB-1 complex
+//   products, relative error ~1e-7 after NB-1 = 3 steps).
+// Spectra (B operand): X_k[c][chunk] for the K-step, staged in LDS in the
+//   planes of the form (float32: re, im, -im; bfloat16: re hi, re lo, im hi,
+//   im lo and the negated im planes; float16: re, im, -im), formed on load from
+//   the packed forward transform (real-input pairing). The next K-step's
+//   spectra and delays are loaded into registers while the current one is
+//   multiplied; one barrier per K-step (two LDS buffers).
+// Complex MAC as real MFMAs: Yr += Pr Xr + Pi (-Xi), Yi += Pr Xi + Pi Xr.
+// Forms: float32 (v_mfma_f32_32x32x2_f32), bfloat16 (v_mfma_f32_32x32x16_bf16,
+// phasors and spectra split into truncated hi + rounded lo, products
+// Ph Xh + Ph Xl + Pl Xh), float16 (v_mfma_f32_32x32x16_f16, one product,
+// spectra scaled by a power of two into the float16 range and scaled back in
+// the epilogue). Float32 accumulation throughout.
+// Operand layout: the k-positions of a K-step are assigned the same way in A
+// and B (lane group g = lane/32, element s -> channel 8g+s), so the sum runs
+// over channels whatever the hardware's k order. Accumulator element r of lane
+// l is row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 #include <hip/hip_runtime.h>
 
 #include "fdd_kernels.h"
@@ -25,62 +38,108 @@
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 
-constexpr int WAVES = 8;   // waves per workgroup, one 32-DM tile each
-constexpr int NB = 4;      // bins per workgroup
-constexpr int KS = 16;     // channels per K-step
-constexpr int NJ = 32;     // chunks per tile
-constexpr int NX = NB * NJ * KS;  // staged X values per K-step
+constexpr int WAVES = 8;               // waves per workgroup, one 32-DM tile each
+constexpr int NB = 4;                  // bins per workgroup
+constexpr int KS = 16;                 // channels per K-step
+constexpr int NJ = 32;                 // chunks per tile
+constexpr int SLAB = NB * 2 * NJ * 8;  // elements of one plane of a K-step: [b][g][j][s]
+constexpr int NT = WAVES * 64;
+static_assert(NB * NJ * KS == NT * 4, "staging: four spectra per thread");
+
+template <int PREC>
+struct Form;
+template <>
+struct Form<0> {  // float32: re, im, -im
+    static constexpr int PLANES = 3, ELEM = 4;
+};
+template <>
+struct Form<1> {  // bfloat16: re hi, re lo, im hi, im lo, -im hi, -im lo
+    static constexpr int PLANES = 6, ELEM = 2;
+};
+template <>
+struct Form<2> {  // float16: re, im, -im
+    static constexpr int PLANES = 3, ELEM = 2;
+};
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// X[k] of the real series from the packed transform (see fft_kernels.hip).
-__device__ __forceinline__ float2 unpack_bin(const float2* __restrict__ zrow, uint32_t k, uint32_t M,
-                                             float2 wk /* e^{-2 pi i k/nfft} */)
+// X[k] of the real series from the packed transform Z (M points):
+// X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]), W^k = e^{-2 pi i k/nfft}.
+__device__ __forceinline__ float2 pair_bin(float2 a, float2 bz, float2 wk)
 {
-    const float2 a = zrow[k & (M - 1)];
-    const float2 bz = zrow[(M - k) & (M - 1)];
     const float2 e = make_float2(0.5f * (a.x + bz.x), 0.5f * (a.y - bz.y));
-    const float2 d = make_float2(a.x - bz.x, a.y + bz.y);
-    const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+    const float2 o = make_float2(0.5f * (a.y + bz.y), -0.5f * (a.x - bz.x));
     const float2 r = cmul(o, wk);
     return make_float2(e.x + r.x, e.y + r.y);
 }
 
-__device__ __forceinline__ uint16_t bf16_trunc_bits(float x)
+__device__ __forceinline__ uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
+
+// A 32-bit constant in a scalar register, out of the compiler's reach. The
+// bfloat16 pair (-1, 0) folded into v_dot2c_f32_bf16 as the inline constant
+// -1.0 is not read as (-1, 0) by gfx950 (measured: every remainder wrong),
+// while the same value from a scalar register is.
+__device__ __forceinline__ uint32_t sconst(uint32_t v)
 {
-    return (uint16_t)(__builtin_bit_cast(uint32_t, x) >> 16);
+    uint32_t r;
+    asm("s_mov_b32 %0, %1" : "=s"(r) : "i"(v));
+    return r;
 }
-__device__ __forceinline__ float bf16_bits_to_float(uint16_t h)
+
+// hi: truncated bfloat16 of a (low half) and b (high half); lo: the rounded
+// bfloat16 of the remainders a - hi(a), b - hi(b), which are exact in float32
+// (formed with a dot product against (-1, 0) / (0, -1)).
+__device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uint32_t& lo)
 {
-    return __builtin_bit_cast(float, (uint32_t)h << 16);
+    hi = __builtin_amdgcn_perm(fbits(b), fbits(a), 0x07060302u);
+    const bf16x2 h = __builtin_bit_cast(bf16x2, hi);
+    const bf16x2 m_lo = __builtin_bit_cast(bf16x2, sconst(0x0000BF80u));
+    const bf16x2 m_hi = __builtin_bit_cast(bf16x2, sconst(0xBF800000u));
+    const float ra = __builtin_amdgcn_fdot2_f32_bf16(h, m_lo, a, false);
+    const float rb = __builtin_amdgcn_fdot2_f32_bf16(h, m_hi, b, false);
+    const bf16x2 l = {(__bf16)ra, (__bf16)rb};
+    lo = __builtin_bit_cast(uint32_t, l);
 }
-__device__ __forceinline__ uint16_t bf16_rne_bits(float x)
+
+// Round-to-nearest float16 pair (a in the low half). Written as the one
+// conversion instruction so the compiler does not fuse it into the producing
+// FMA (which then has to be computed twice).
+__device__ __forceinline__ uint32_t pack_f16x2(float a, float b)
 {
-    return __builtin_bit_cast(uint16_t, (__bf16)x);
+    uint32_t r;
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
-__device__ __forceinline__ uint16_t f16_bits(float x)
+
+// e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used): LDS byte offsets of the
+// two table entries are (m8 >> 8) & 0x7f8 and m8 & 0x7f8.
+__device__ __forceinline__ float2 phasor(const char* tab, uint32_t m8)
 {
-    return __builtin_bit_cast(uint16_t, (_Float16)x);
+    const float2 hi = *reinterpret_cast<const float2*>(tab + ((m8 >> 8) & 0x7f8u));
+    const float2 lo = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
+    return cmul(hi, lo);
 }
 
 template <int PREC>
-__global__ __launch_bounds__(WAVES * 64) void fdd_mfma_k(
-    const float2* __restrict__ Z, size_t Zp, const int32_t* __restrict__ delays, uint32_t ndm,
-    uint32_t nchan, uint32_t nchunk, const float2* __restrict__ ph, uint32_t nfft,
+__global__ __launch_bounds__(NT) void fdd_mfma_k(
+    const float2* __restrict__ Z, size_t Zp, const uint16_t* __restrict__ q16, size_t qpitch,
+    uint32_t ndm, uint32_t nchan, uint32_t nchunk, const float2* __restrict__ tab65536,
+    const float2* __restrict__ ph, uint32_t nfft, float xscale, float yscale,
     float2* __restrict__ Y, size_t Yp, uint32_t nbb, uint32_t ndb)
 {
-    // X staging planes: fp32 -> 2 planes of float; bf16 -> 4 planes of
-    // 16-bit (re hi, re lo, im hi, im lo); fp16 -> 2 planes of 16-bit.
-    constexpr int PLANES = (PREC == 1) ? 4 : 2;
-    constexpr int ELEM = (PREC == 0) ? 4 : 2;
-    __shared__ __attribute__((aligned(16))) unsigned char xs_raw[PLANES * NX * ELEM];
-    __shared__ float2 tw_hi[256], tw_lo[256];
+    using F = Form<PREC>;
+    constexpr int BUF = F::PLANES * SLAB * F::ELEM;  // bytes of one staged K-step
+    __shared__ __attribute__((aligned(16))) char xs[2 * BUF];
+    __shared__ __attribute__((aligned(16))) char tab[4096];  // T_hi[256], T_lo[256]
 
     const uint32_t M = nfft / 2;
     const int tid = threadIdx.x;
@@ -99,14 +158,79 @@ __global__ __launch_bounds__(WAVES * 64) void fdd_mfma_k(
     const uint32_t k0 = bblk * NB;
     const uint32_t ch0 = cgrp * NJ;
 
-    for (int t = tid; t < 256; t += WAVES * 64) {
-        tw_lo[t] = ph[t & (nfft - 1)];
-        tw_hi[t] = ph[(t * 256) & (nfft - 1)];
-    }
+    for (int t = tid; t < 512; t += NT)
+        reinterpret_cast<float2*>(tab)[t] = tab65536[t];
 
+    // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
+    const int sb = tid & 3, sj = (tid >> 2) & 31, sg = (tid >> 7) & 1, sh = tid >> 8;
+    const uint32_t sk = k0 + sb, sch = ch0 + sj;
+    const bool s_ok = sk <= M && sch < nchunk;
+    const float2 pk = ph[sk & (nfft - 1)];
+    const float2 wk = make_float2(pk.x, -pk.y);  // e^{-2 pi i k/nfft}
+    const int s_off = ((sb * 2 + sg) * NJ + sj) * 8 + 4 * sh;  // element offset in a plane
+    // rows of channels 8g + 4h + q of the K-step: zlo/zhi point at bins k and
+    // M-k of channel 8g + 4h; channel q further is q*rs on, a K-step 16*rs
+    const size_t rs = (size_t)nchunk * Zp;
+    const float2* zrow = Z + ((size_t)(8 * sg + 4 * sh) * nchunk + min(sch, nchunk - 1)) * Zp;
+    const float2* zlo = zrow + (sk & (M - 1));
+    const float2* zhi = zrow + ((M - sk) & (M - 1));
+    uint32_t c_first = 8 * sg + 4 * sh;  // channel of q = 0 in the K-step being loaded
+
+    float2 za[4], zb[4];
+    auto load_z = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            za[q] = zb[q] = make_float2(0.f, 0.f);
+            if (s_ok && c_first + q < nchan) {
+                za[q] = zlo[q * rs];
+                zb[q] = zhi[q * rs];
+            }
+        }
+        zlo += KS * rs;
+        zhi += KS * rs;
+        c_first += KS;
+    };
+    auto store_x = [&](char* buf) {
+        float xr[4], xi[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 x = pair_bin(za[q], zb[q], wk);
+            xr[q] = x.x * xscale;
+            xi[q] = x.y * xscale;
+        }
+        if constexpr (PREC == 0) {
+            float4* p = reinterpret_cast<float4*>(buf) + s_off / 4;
+            p[0] = make_float4(xr[0], xr[1], xr[2], xr[3]);
+            p[SLAB / 4] = make_float4(xi[0], xi[1], xi[2], xi[3]);
+            p[2 * SLAB / 4] = make_float4(-xi[0], -xi[1], -xi[2], -xi[3]);
+        } else if constexpr (PREC == 1) {
+            uint32_t rh0, rl0, rh1, rl1, ih0, il0, ih1, il1;
+            split_bf16x2(xr[0], xr[1], rh0, rl0);
+            split_bf16x2(xr[2], xr[3], rh1, rl1);
+            split_bf16x2(xi[0], xi[1], ih0, il0);
+            split_bf16x2(xi[2], xi[3], ih1, il1);
+            uint2* p = reinterpret_cast<uint2*>(buf) + s_off / 4;
+            p[0] = make_uint2(rh0, rh1);
+            p[SLAB / 4] = make_uint2(rl0, rl1);
+            p[2 * SLAB / 4] = make_uint2(ih0, ih1);
+            p[3 * SLAB / 4] = make_uint2(il0, il1);
+            p[4 * SLAB / 4] = make_uint2(ih0 ^ 0x80008000u, ih1 ^ 0x80008000u);
+            p[5 * SLAB / 4] = make_uint2(il0 ^ 0x80008000u, il1 ^ 0x80008000u);
+        } else {
+            const uint32_t r0 = pack_f16x2(xr[0], xr[1]), r1 = pack_f16x2(xr[2], xr[3]);
+            const uint32_t i0 = pack_f16x2(xi[0], xi[1]), i1 = pack_f16x2(xi[2], xi[3]);
+            uint2* p = reinterpret_cast<uint2*>(buf) + s_off / 4;
+            p[0] = make_uint2(r0, r1);
+            p[SLAB / 4] = make_uint2(i0, i1);
+            p[2 * SLAB / 4] = make_uint2(i0 ^ 0x80008000u, i1 ^ 0x80008000u);
+        }
+    };
+
+    // ---- compute role: DM row i, channel group g ----
     const int i = lane & 31, g = lane >> 5;
     const uint32_t dm = dblk * (WAVES * 32) + wave * 32 + i;
-    const int32_t* drow = delays + (size_t)min(dm, ndm - 1) * nchan;
+    const uint16_t* qrow = q16 + (size_t)min(dm, ndm - 1) * qpitch + 8 * g;
+    const int b_off = (g * NJ + i) * 8;  // B operand: chunk j = lane%32 (same index as i)
 
     f32x16 acc_r[NB], acc_i[NB];
 #pragma unroll
@@ -114,158 +238,161 @@ __global__ __launch_bounds__(WAVES * 64) void fdd_mfma_k(
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc_r[b][r] = acc_i[b][r] = 0.f;
 
-    // e^{-2 pi i k/nfft} of the bins this thread stages (bin = tid % NB)
-    const uint32_t kst = k0 + (tid % NB);
-    const float2 pkst = ph[kst & (nfft - 1)];
-    const float2 wst = make_float2(pkst.x, -pkst.y);
-
     const uint32_t nks = (nchan + KS - 1) / KS;
-    for (uint32_t ks = 0; ks < nks; ++ks) {
-        const uint32_t c0 = ks * KS;
-        __syncthreads();
-        // ---- stage X for NB bins x NJ chunks x KS channels ----
-#pragma unroll
-        for (int r = 0; r < NX / (WAVES * 64); ++r) {
-            const int e = tid + r * WAVES * 64;
-            const int b = e % NB, j = (e / NB) % NJ, cl = e / (NB * NJ);
-            const uint32_t c = c0 + cl, ch = ch0 + j;
-            float2 x = make_float2(0.f, 0.f);
-            if (c < nchan && ch < nchunk && kst <= M)
-                x = unpack_bin(Z + ((size_t)c * nchunk + ch) * Zp, kst, M, wst);
-            const int idx = (b * NJ + j) * KS + cl;
-            if constexpr (PREC == 0) {
-                float* xs = reinterpret_cast<float*>(xs_raw);
-                xs[idx] = x.x;
-                xs[NX + idx] = x.y;
-            } else if constexpr (PREC == 1) {
-                uint16_t* xs = reinterpret_cast<uint16_t*>(xs_raw);
-                const uint16_t rh = bf16_trunc_bits(x.x), ih = bf16_trunc_bits(x.y);
-                xs[idx] = rh;
-                xs[NX + idx] = bf16_rne_bits(x.x - bf16_bits_to_float(rh));
-                xs[2 * NX + idx] = ih;
-                xs[3 * NX + idx] = bf16_rne_bits(x.y - bf16_bits_to_float(ih));
-            } else {
-                uint16_t* xs = reinterpret_cast<uint16_t*>(xs_raw);
-                xs[idx] = f16_bits(x.x);
-                xs[NX + idx] = f16_bits(x.y);
-            }
-        }
-        __syncthreads();
+    load_z();
+    u32x4 qv = *reinterpret_cast<const u32x4*>(qrow);
+    store_x(xs);
+    __syncthreads();
 
-        // ---- delays of this lane's (DM, 8 channels) ----
-        int32_t d[8];
+    for (uint32_t ks = 0; ks < nks; ++ks) {
+        const char* cur = xs + (ks & 1) * BUF;
+        const bool more = ks + 1 < nks;
+        u32x4 qn = qv;
+        if (more) {
+            load_z();
+            qn = *reinterpret_cast<const u32x4*>(qrow + (ks + 1) * KS);
+        }
+
+        // phasors of bin k0 and the per-bin steps for this lane's 8 channels
+        float2 p[8], w[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const uint32_t c = c0 + 8 * g + s;
-            d[s] = (c < nchan) ? drow[c] : 0;
+            const uint32_t q = (qv[s >> 1] >> (16 * (s & 1))) & 0xffffu;
+            p[s] = phasor(tab, __umul24(8u * k0, q));
+            w[s] = phasor(tab, q << 3);
         }
 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            const uint32_t k = k0 + b;
-            float2 p[8];
+            const char* bb = cur + (size_t)(b * 2 * NJ * 8 + b_off) * F::ELEM;
+            // phasors of the next bin first: the operand formation below may
+            // then overwrite p in place
+            float2 pn[8];
+            if (b + 1 < NB) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const uint32_t m = (k * (uint32_t)d[s]) & (nfft - 1);
-                p[s] = cmul(tw_hi[m >> 8], tw_lo[m & 255]);
+                for (int s = 0; s < 8; ++s) pn[s] = cmul(p[s], w[s]);
             }
-            const int base = (b * NJ + i) * KS + 8 * g;  // B operand: chunk j = lane%32
             if constexpr (PREC == 0) {
-                const float* xs = reinterpret_cast<const float*>(xs_raw);
+                const float4* X = reinterpret_cast<const float4*>(bb);
+                float xr[8], xi[8], xn[8];
+                *reinterpret_cast<float4*>(xr) = X[0];
+                *reinterpret_cast<float4*>(xr + 4) = X[1];
+                *reinterpret_cast<float4*>(xi) = X[SLAB / 4];
+                *reinterpret_cast<float4*>(xi + 4) = X[SLAB / 4 + 1];
+                *reinterpret_cast<float4*>(xn) = X[2 * SLAB / 4];
+                *reinterpret_cast<float4*>(xn + 4) = X[2 * SLAB / 4 + 1];
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
-                    const float xr = xs[base + s], xi = xs[NX + base + s];
-                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xr, acc_r[b], 0, 0, 0);
-                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(-p[s].y, xi, acc_r[b], 0, 0, 0);
-                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xi, acc_i[b], 0, 0, 0);
-                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xr, acc_i[b], 0, 0, 0);
+                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xr[s], acc_r[b], 0, 0, 0);
+                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xn[s], acc_r[b], 0, 0, 0);
+                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xi[s], acc_i[b], 0, 0, 0);
+                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xr[s], acc_i[b], 0, 0, 0);
                 }
             } else if constexpr (PREC == 1) {
-                const uint16_t* xs = reinterpret_cast<const uint16_t*>(xs_raw);
-                const bf16x8 xrh = *reinterpret_cast<const bf16x8*>(xs + base);
-                const bf16x8 xrl = *reinterpret_cast<const bf16x8*>(xs + NX + base);
-                const bf16x8 xih = *reinterpret_cast<const bf16x8*>(xs + 2 * NX + base);
-                const bf16x8 xil = *reinterpret_cast<const bf16x8*>(xs + 3 * NX + base);
-                bf16x8 prh, prl, pih, pil, nih, nil;
+                const bf16x8* X = reinterpret_cast<const bf16x8*>(bb);
+                uint32_t prh[4], prl[4], pih[4], pil[4];
 #pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    const uint16_t rh = bf16_trunc_bits(p[s].x), ih = bf16_trunc_bits(p[s].y);
-                    const uint16_t rl = bf16_rne_bits(p[s].x - bf16_bits_to_float(rh));
-                    const uint16_t il = bf16_rne_bits(p[s].y - bf16_bits_to_float(ih));
-                    prh[s] = __builtin_bit_cast(__bf16, rh);
-                    prl[s] = __builtin_bit_cast(__bf16, rl);
-                    pih[s] = __builtin_bit_cast(__bf16, ih);
-                    pil[s] = __builtin_bit_cast(__bf16, il);
-                    nih[s] = __builtin_bit_cast(__bf16, (uint16_t)(ih ^ 0x8000u));
-                    nil[s] = __builtin_bit_cast(__bf16, (uint16_t)(il ^ 0x8000u));
+                for (int s = 0; s < 4; ++s) {
+                    split_bf16x2(p[2 * s].x, p[2 * s + 1].x, prh[s], prl[s]);
+                    split_bf16x2(p[2 * s].y, p[2 * s + 1].y, pih[s], pil[s]);
                 }
+                const bf16x8 Arh = __builtin_bit_cast(bf16x8, prh), Arl = __builtin_bit_cast(bf16x8, prl);
+                const bf16x8 Aih = __builtin_bit_cast(bf16x8, pih), Ail = __builtin_bit_cast(bf16x8, pil);
                 f32x16 ar = acc_r[b], ai = acc_i[b];
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, xrh, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, xrl, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl, xrh, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, xih, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nih, xil, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nil, xih, ar, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, xih, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prh, xil, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(prl, xih, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, xrh, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pih, xrl, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pil, xrh, ai, 0, 0, 0);
+                // grouped by spectra plane (each read once, used, then dead)
+                bf16x8 x = X[0];  // re hi
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ai, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, x, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ail, x, ai, 0, 0, 0);
+                x = X[SLAB / 8];  // re lo
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ai, 0, 0, 0);
+                x = X[2 * SLAB / 8];  // im hi
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, x, ai, 0, 0, 0);
+                x = X[4 * SLAB / 8];  // -im hi
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ail, x, ar, 0, 0, 0);
+                x = X[3 * SLAB / 8];  // im lo
+                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                x = X[5 * SLAB / 8];  // -im lo
+                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
                 acc_r[b] = ar;
                 acc_i[b] = ai;
             } else {
-                const uint16_t* xs = reinterpret_cast<const uint16_t*>(xs_raw);
-                const f16x8 xr = *reinterpret_cast<const f16x8*>(xs + base);
-                const f16x8 xi = *reinterpret_cast<const f16x8*>(xs + NX + base);
-                f16x8 pr, pi, ni;
+                const f16x8* X = reinterpret_cast<const f16x8*>(bb);
+                const f16x8 xr = X[0], xi = X[SLAB / 8], xn = X[2 * SLAB / 8];
+                uint32_t pr[4], pi[4];
 #pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    pr[s] = (_Float16)p[s].x;
-                    pi[s] = (_Float16)p[s].y;
-                    ni[s] = -pi[s];
+                for (int s = 0; s < 4; ++s) {
+                    pr[s] = pack_f16x2(p[2 * s].x, p[2 * s + 1].x);
+                    pi[s] = pack_f16x2(p[2 * s].y, p[2 * s + 1].y);
                 }
-                acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr, xr, acc_r[b], 0, 0, 0);
-                acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ni, xi, acc_r[b], 0, 0, 0);
-                acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr, xi, acc_i[b], 0, 0, 0);
-                acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi, xr, acc_i[b], 0, 0, 0);
+                const f16x8 Ar = __builtin_bit_cast(f16x8, pr), Ai = __builtin_bit_cast(f16x8, pi);
+                acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ar, xr, acc_r[b], 0, 0, 0);
+                acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ar, xi, acc_i[b], 0, 0, 0);
+                acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, xn, acc_r[b], 0, 0, 0);
+                acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, xr, acc_i[b], 0, 0, 0);
+            }
+            if (b + 1 < NB) {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) p[s] = pn[s];
             }
         }
+
+        if (more) store_x(xs + ((ks + 1) & 1) * BUF);
+        qv = qn;
+        __syncthreads();
     }
 
-    // ---- epilogue: 32x32 accumulator layout, column j = lane%32,
-    // row = 8*(r/4) + 4*(lane/32) + r%4 for element r ----
+    // ---- epilogue: rows 8*(r/4) + 4*g + r%4 of the tile, column = chunk ----
     const uint32_t ch = ch0 + i;
+    if (ch >= nchunk) return;
+    const bool all_bins = k0 + NB - 1 <= M;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const uint32_t k = k0 + b;
-        if (k > M || ch >= nchunk) continue;
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t row = 8 * (r / 4) + 4 * g + (r % 4);
+        const uint32_t dmo = dblk * (WAVES * 32) + wave * 32 + row;
+        if (dmo >= ndm) continue;
+        float2* dst = Y + ((size_t)dmo * nchunk + ch) * Yp + k0;
+        if (all_bins) {
+            float4* d4 = reinterpret_cast<float4*>(dst);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t row = 8 * (r / 4) + 4 * g + (r % 4);
-            const uint32_t dmo = dblk * (WAVES * 32) + wave * 32 + row;
-            if (dmo < ndm) Y[((size_t)dmo * nchunk + ch) * Yp + k] = make_float2(acc_r[b][r], acc_i[b][r]);
+            for (int b = 0; b < NB; b += 2)
+                d4[b / 2] = make_float4(acc_r[b][r] * yscale, acc_i[b][r] * yscale,
+                                        acc_r[b + 1][r] * yscale, acc_i[b + 1][r] * yscale);
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                if (k0 + b <= M) dst[b] = make_float2(acc_r[b][r] * yscale, acc_i[b][r] * yscale);
         }
     }
 }
 
 }  // namespace
 
-extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const int32_t* delays,
-                                           size_t ndm, size_t nchan, size_t nchunk, const float2* ph,
-                                           size_t nfft, float2* Y, size_t Yp, hipStream_t stream)
+extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const uint16_t* q16,
+                                           size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
+                                           const float2* tab65536, const float2* ph, size_t nfft,
+                                           float xscale, float2* Y, size_t Yp, hipStream_t stream)
 {
     if (ndm == 0 || nchunk == 0) return hipSuccess;
+    // layout preconditions of the vector loads and stores
+    if (qpitch % 16 || qpitch < (nchan + 15) / 16 * 16 || Yp % 2 || nfft < 2 * NB || nfft > 65536 ||
+        (nfft & (nfft - 1)))
+        return hipErrorInvalidValue;
     const size_t M = nfft / 2;
     const uint32_t nbb = (uint32_t)((M + 1 + NB - 1) / NB);
     const uint32_t ndb = (uint32_t)((ndm + WAVES * 32 - 1) / (WAVES * 32));
     const uint32_t ncg = (uint32_t)((nchunk + NJ - 1) / NJ);
     const size_t nblocks = (size_t)nbb * ndb * ncg;
     if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;
-#define DD_MFMA(P)                                                                                \
-    hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(WAVES * 64), 0, stream, Z, Zp, \
-                       delays, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, ph, (uint32_t)nfft, \
-                       Y, Yp, nbb, ndb)
+    const float yscale = 1.0f / xscale;
+#define DD_MFMA(P)                                                                                  \
+    hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, Z, Zp, q16,     \
+                       qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536, ph,      \
+                       (uint32_t)nfft, xscale, yscale, Y, Yp, nbb, ndb)
     switch (prec) {
     case 0: DD_MFMA(0); break;
     case 1: DD_MFMA(1); break;

@@ -38,6 +38,8 @@ struct FDDState {
     DeviceBuffer d_Y;       // [dm batch][nchunk][Yp] dedispersed spectra
     DeviceBuffer d_out;     // staged output (host-pointer mode)
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
+    DeviceBuffer d_q16;     // [ndm][qpitch] uint16 phase steps d*65536/nfft mod 65536
+    DeviceBuffer d_tab;     // [512] e^{2 pi i 256a/65536}, e^{2 pi i b/65536}
     DeviceBuffer d_scale;   // [nchan] per-channel input scale
     DeviceBuffer d_ph;      // [nfft] e^{+2 pi i m / nfft}
 
@@ -52,6 +54,10 @@ struct FDDState {
     unsigned long version = ~0ul;
     int32_t dmin = 0, dmax = 0;
     size_t ph_nfft = 0;
+    unsigned long q16_version = ~0ul;
+    size_t q16_nfft = 0, qpitch = 0;
+    std::vector<int32_t> h_delays;  // host copy of d_delays
+    float max_scale = 0.f;
 };
 
 namespace {
@@ -170,6 +176,8 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
         DD_HIP(hipStreamSynchronize(stream));
         st.dmin = dmin;
         st.dmax = dmax;
+        st.h_delays.swap(d);
+        st.max_scale = nchan ? *std::max_element(scale.begin(), scale.end()) : 0.f;
         st.version = m_tables_version;
     }
     if (st.dmin < 0) DD_THROW(DEDISP_UNKNOWN_ERROR);  // negative DMs: not supported
@@ -207,6 +215,41 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
         st.ph_nfft = nfft;
     }
     const float2* ph = st.d_ph.as<float2>();
+
+    // phase steps of the matrix-core contraction (65536-based phases)
+    const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
+    if (!valu && (st.q16_version != m_tables_version || st.q16_nfft != nfft)) {
+        const size_t qpitch = round_up(nchan, 16);
+        const uint32_t mul = (uint32_t)(65536 / nfft);
+        std::vector<uint16_t> q(ndm * qpitch, 0);
+        for (size_t i = 0; i < ndm; ++i)
+            for (size_t c = 0; c < nchan; ++c)
+                q[i * qpitch + c] = (uint16_t)(((uint32_t)st.h_delays[i * nchan + c] * mul) & 0xffffu);
+        std::vector<float> tab(1024);
+        for (int a = 0; a < 256; ++a) {
+            const double t1 = 2.0 * M_PI * (double)(256 * a) / 65536.0;
+            const double t2 = 2.0 * M_PI * (double)a / 65536.0;
+            tab[2 * a] = (float)std::cos(t1);
+            tab[2 * a + 1] = (float)std::sin(t1);
+            tab[512 + 2 * a] = (float)std::cos(t2);
+            tab[512 + 2 * a + 1] = (float)std::sin(t2);
+        }
+        st.d_q16.reserve(q.size() * 2);
+        st.d_tab.reserve(tab.size() * 4);
+        DD_HIP(hipMemcpyAsync(st.d_q16.ptr, q.data(), q.size() * 2, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_tab.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.qpitch = qpitch;
+        st.q16_nfft = nfft;
+        st.q16_version = m_tables_version;
+    }
+    // float16 spectra: |X| <= nfft * 127.5 * max scale; a power of two brings
+    // that bound to at most 2^15 (float16 max 65504)
+    float xscale = 1.f;
+    if (st.precision == DEDISP_AMD_FDD_FP16 && st.max_scale > 0.f) {
+        const double bound = (double)nfft * 127.5 * (double)st.max_scale;
+        xscale = (float)std::ldexp(1.0, (int)std::floor(std::log2(32768.0 / bound)));
+    }
 
     // ---- input ----
     const uint8_t* d_in;
@@ -277,7 +320,6 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
     }
     // DEDISP_FDD_VALU=1 selects the float32 vector-ALU contraction instead of
     // the matrix cores (same arithmetic as the float32 MFMA form)
-    const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
     for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
         const size_t nb = std::min(dm_batch, ndm - b0);
         ctx.timer.begin(StageTimer::DEDISPERSE, stream);
@@ -286,8 +328,10 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
             DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Zp, dl, nb, nchan, nchunk, ph, nfft,
                                         st.d_Y.as<float2>(), Yp, stream));
         else
-            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Zp, dl, nb, nchan,
-                                        nchunk, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
+            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Zp,
+                                        st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan,
+                                        nchunk, st.d_tab.as<float2>(), ph, nfft, xscale,
+                                        st.d_Y.as<float2>(), Yp, stream));
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
         ctx.timer.begin(StageTimer::IFFT, stream);
         if (!fourier) {
