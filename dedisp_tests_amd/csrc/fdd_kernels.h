@@ -55,6 +55,7 @@ hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delay
 // tab65536 holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
 // (b < 256). The spectra are multiplied by xscale before the MFMA inputs are
 // formed and Y by 1/xscale after (a power of two keeping float16 in range).
+// Z must hold nchan rounded up to 16 channel rows, the extra rows zero.
 hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const uint16_t* q16,
                                 size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
                                 const float2* tab65536, const float2* ph, size_t nfft,

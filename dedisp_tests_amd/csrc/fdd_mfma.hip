@@ -44,13 +44,22 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 
-constexpr int WAVES = 8;               // waves per workgroup, one 32-DM tile each
+#ifndef DD_MFMA_STAGGER
+#define DD_MFMA_STAGGER 1
+#endif
+
+#ifndef DD_MFMA_WAVES
+#define DD_MFMA_WAVES 8
+#endif
+
+constexpr int WAVES = DD_MFMA_WAVES;   // waves per workgroup, one 32-DM tile each
 constexpr int NB = 4;                  // bins per workgroup
 constexpr int KS = 16;                 // channels per K-step
 constexpr int NJ = 32;                 // chunks per tile
 constexpr int SLAB = NB * 2 * NJ * 8;  // elements of one plane of a K-step: [b][g][j][s]
 constexpr int NT = WAVES * 64;
-static_assert(NB * NJ * KS == NT * 4, "staging: four spectra per thread");
+constexpr int HG = NB * NJ * KS / (4 * NT);  // groups of 4 channels a thread stages
+static_assert(HG >= 1 && HG * 4 * NT == NB * NJ * KS, "staging layout");
 
 template <int PREC>
 struct Form;
@@ -72,14 +81,15 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// X[k] of the real series from the packed transform Z (M points):
-// X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]), W^k = e^{-2 pi i k/nfft}.
-__device__ __forceinline__ float2 pair_bin(float2 a, float2 bz, float2 wk)
+// h*X[k] of the real series from the packed transform Z (M points):
+// X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]), W^k = e^{-2 pi i k/nfft};
+// hw = h/2 * W^k, hh = h/2 (h = 0 gives exactly 0 for finite Z).
+__device__ __forceinline__ float2 pair_bin(float2 a, float2 bz, float2 hw, float hh)
 {
-    const float2 e = make_float2(0.5f * (a.x + bz.x), 0.5f * (a.y - bz.y));
-    const float2 o = make_float2(0.5f * (a.y + bz.y), -0.5f * (a.x - bz.x));
-    const float2 r = cmul(o, wk);
-    return make_float2(e.x + r.x, e.y + r.y);
+    const float ux = a.x + bz.x, uy = a.y - bz.y;  // Z[k] + conj Z[M-k]
+    const float vx = a.x - bz.x, vy = a.y + bz.y;  // Z[k] - conj Z[M-k]
+    const float tx = vx * hw.x - vy * hw.y, ty = vx * hw.y + vy * hw.x;
+    return make_float2(fmaf(hh, ux, ty), fmaf(hh, uy, -tx));
 }
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
@@ -120,17 +130,16 @@ __device__ __forceinline__ uint32_t pack_f16x2(float a, float b)
     return r;
 }
 
-// e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used): LDS byte offsets of the
-// two table entries are (m8 >> 8) & 0x7f8 and m8 & 0x7f8.
-__device__ __forceinline__ float2 phasor(const char* tab, uint32_t m8)
-{
-    const float2 hi = *reinterpret_cast<const float2*>(tab + ((m8 >> 8) & 0x7f8u));
-    const float2 lo = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
-    return cmul(hi, lo);
-}
+// e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used) is T_hi[m>>8] * T_lo[m&255];
+// the LDS byte offsets of the two entries are (m8 >> 8) & 0x7f8 and
+// 2048 + (m8 & 0x7f8).
 
 template <int PREC>
-__global__ __launch_bounds__(NT) void fdd_mfma_k(
+__global__ __launch_bounds__(NT)
+#if DD_MFMA_OCC2
+__attribute__((amdgpu_waves_per_eu(2, 2)))
+#endif
+void fdd_mfma_k(
     const float2* __restrict__ Z, size_t Zp, const uint16_t* __restrict__ q16, size_t qpitch,
     uint32_t ndm, uint32_t nchan, uint32_t nchunk, const float2* __restrict__ tab65536,
     const float2* __restrict__ ph, uint32_t nfft, float xscale, float yscale,
@@ -162,44 +171,46 @@ __global__ __launch_bounds__(NT) void fdd_mfma_k(
         reinterpret_cast<float2*>(tab)[t] = tab65536[t];
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
-    const int sb = tid & 3, sj = (tid >> 2) & 31, sg = (tid >> 7) & 1, sh = tid >> 8;
+    // Every load is unconditional and in bounds (Z has its channel count
+    // padded to a multiple of 16 with zero rows; out-of-range bins and chunks
+    // read a valid row and are multiplied by 0 in the pairing), so the
+    // compiler has no reason to wait for the loads before they are consumed
+    // at the end of the K-step.
+    const int sb = tid & 3, sj = (tid >> 2) & 31, sg = (tid >> 7) & 1, sh0 = (tid >> 8) * HG;
     const uint32_t sk = k0 + sb, sch = ch0 + sj;
-    const bool s_ok = sk <= M && sch < nchunk;
+    const float hh = (sk <= M && sch < nchunk) ? 0.5f * xscale : 0.f;
     const float2 pk = ph[sk & (nfft - 1)];
-    const float2 wk = make_float2(pk.x, -pk.y);  // e^{-2 pi i k/nfft}
-    const int s_off = ((sb * 2 + sg) * NJ + sj) * 8 + 4 * sh;  // element offset in a plane
+    const float2 hw = make_float2(hh * pk.x, -hh * pk.y);  // h/2 e^{-2 pi i k/nfft}
+    const int s_off = ((sb * 2 + sg) * NJ + sj) * 8 + 4 * sh0;  // element offset in a plane
     // rows of channels 8g + 4h + q of the K-step: zlo/zhi point at bins k and
-    // M-k of channel 8g + 4h; channel q further is q*rs on, a K-step 16*rs
+    // M-k of channel 8g + 4*h0; channel 4u + q further is (4u + q)*rs on, a
+    // K-step 16*rs
     const size_t rs = (size_t)nchunk * Zp;
-    const float2* zrow = Z + ((size_t)(8 * sg + 4 * sh) * nchunk + min(sch, nchunk - 1)) * Zp;
+    const float2* zrow = Z + ((size_t)(8 * sg + 4 * sh0) * nchunk + min(sch, nchunk - 1)) * Zp;
     const float2* zlo = zrow + (sk & (M - 1));
     const float2* zhi = zrow + ((M - sk) & (M - 1));
-    uint32_t c_first = 8 * sg + 4 * sh;  // channel of q = 0 in the K-step being loaded
 
-    float2 za[4], zb[4];
+    float2 za[4 * HG], zb[4 * HG];
     auto load_z = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            za[q] = zb[q] = make_float2(0.f, 0.f);
-            if (s_ok && c_first + q < nchan) {
-                za[q] = zlo[q * rs];
-                zb[q] = zhi[q * rs];
-            }
+        for (int q = 0; q < 4 * HG; ++q) {
+            za[q] = zlo[q * rs];
+            zb[q] = zhi[q * rs];
         }
-        zlo += KS * rs;
-        zhi += KS * rs;
-        c_first += KS;
     };
     auto store_x = [&](char* buf) {
+#pragma unroll
+      for (int u = 0; u < HG; ++u) {
         float xr[4], xi[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 x = pair_bin(za[q], zb[q], wk);
-            xr[q] = x.x * xscale;
-            xi[q] = x.y * xscale;
+            const float2 x = pair_bin(za[4 * u + q], zb[4 * u + q], hw, hh);
+            xr[q] = x.x;
+            xi[q] = x.y;
         }
+        const int off = s_off + 4 * u;
         if constexpr (PREC == 0) {
-            float4* p = reinterpret_cast<float4*>(buf) + s_off / 4;
+            float4* p = reinterpret_cast<float4*>(buf) + off / 4;
             p[0] = make_float4(xr[0], xr[1], xr[2], xr[3]);
             p[SLAB / 4] = make_float4(xi[0], xi[1], xi[2], xi[3]);
             p[2 * SLAB / 4] = make_float4(-xi[0], -xi[1], -xi[2], -xi[3]);
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(NT) void fdd_mfma_k(
             split_bf16x2(xr[2], xr[3], rh1, rl1);
             split_bf16x2(xi[0], xi[1], ih0, il0);
             split_bf16x2(xi[2], xi[3], ih1, il1);
-            uint2* p = reinterpret_cast<uint2*>(buf) + s_off / 4;
+            uint2* p = reinterpret_cast<uint2*>(buf) + off / 4;
             p[0] = make_uint2(rh0, rh1);
             p[SLAB / 4] = make_uint2(rl0, rl1);
             p[2 * SLAB / 4] = make_uint2(ih0, ih1);
@@ -219,11 +230,12 @@ __global__ __launch_bounds__(NT) void fdd_mfma_k(
         } else {
             const uint32_t r0 = pack_f16x2(xr[0], xr[1]), r1 = pack_f16x2(xr[2], xr[3]);
             const uint32_t i0 = pack_f16x2(xi[0], xi[1]), i1 = pack_f16x2(xi[2], xi[3]);
-            uint2* p = reinterpret_cast<uint2*>(buf) + s_off / 4;
+            uint2* p = reinterpret_cast<uint2*>(buf) + off / 4;
             p[0] = make_uint2(r0, r1);
             p[SLAB / 4] = make_uint2(i0, i1);
             p[2 * SLAB / 4] = make_uint2(i0 ^ 0x80008000u, i1 ^ 0x80008000u);
         }
+      }
     };
 
     // ---- compute role: DM row i, channel group g ----
@@ -238,29 +250,52 @@ __global__ __launch_bounds__(NT) void fdd_mfma_k(
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc_r[b][r] = acc_i[b][r] = 0.f;
 
+    // Phasors of bin k0 and the per-bin steps for this lane's 8 channels. The
+    // LDS table reads of a K-step are issued before any product uses them.
+    // The second half of the waves computes them for the next K-step at the
+    // end of the current one (the registers are free by then), so after each
+    // barrier one wave of a SIMD issues MFMAs while the other waits on LDS.
+    float2 p[8], w[8];
+    auto lookups = [&](const u32x4& qq) {
+        float2 th[8], tl[8], wh[8], wl[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t q = (qq[s >> 1] >> (16 * (s & 1))) & 0xffffu;
+            const uint32_t m8 = __umul24(8u * k0, q), w8 = q << 3;
+            th[s] = *reinterpret_cast<const float2*>(tab + ((m8 >> 8) & 0x7f8u));
+            tl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
+            wh[s] = *reinterpret_cast<const float2*>(tab + ((w8 >> 8) & 0x7f8u));
+            wl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (w8 & 0x7f8u));
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            p[s] = cmul(th[s], tl[s]);
+            w[s] = cmul(wh[s], wl[s]);
+        }
+    };
+    const bool late = DD_MFMA_STAGGER && WAVES == 8 && wave >= WAVES / 2;
+
     const uint32_t nks = (nchan + KS - 1) / KS;
+    const size_t kstep = KS * rs;
     load_z();
     u32x4 qv = *reinterpret_cast<const u32x4*>(qrow);
     store_x(xs);
+    if (late) lookups(qv);
+    if (nks > 1) zlo += kstep, zhi += kstep;
     __syncthreads();
 
     for (uint32_t ks = 0; ks < nks; ++ks) {
         const char* cur = xs + (ks & 1) * BUF;
         const bool more = ks + 1 < nks;
-        u32x4 qn = qv;
-        if (more) {
-            load_z();
-            qn = *reinterpret_cast<const u32x4*>(qrow + (ks + 1) * KS);
-        }
+        // prefetch K-step min(ks + 1, nks - 1): the last iteration re-reads
+        // its own K-step (in bounds, unused)
+        load_z();
+        const u32x4 qn = *reinterpret_cast<const u32x4*>(qrow + min(ks + 1, nks - 1) * KS);
+        const size_t inc = ks + 2 < nks ? kstep : 0;
+        zlo += inc;
+        zhi += inc;
 
-        // phasors of bin k0 and the per-bin steps for this lane's 8 channels
-        float2 p[8], w[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t q = (qv[s >> 1] >> (16 * (s & 1))) & 0xffffu;
-            p[s] = phasor(tab, __umul24(8u * k0, q));
-            w[s] = phasor(tab, q << 3);
-        }
+        if (!late) lookups(qv);
 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
@@ -341,6 +376,7 @@ __global__ __launch_bounds__(NT) void fdd_mfma_k(
             }
         }
 
+        if (late && more) lookups(qn);
         if (more) store_x(xs + ((ks + 1) & 1) * BUF);
         qv = qn;
         __syncthreads();

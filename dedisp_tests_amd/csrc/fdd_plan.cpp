@@ -293,7 +293,12 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
     const size_t in_span = fourier ? nsamps : std::max(nsamps, (nchunk - 1) * good + nfft);
     const size_t row_words = div_up(in_span, 4 * 64) * 64;
     st.d_rows.reserve(nchan * row_words * 4);
-    st.d_Z.reserve(nchan * nchunk * Zp * 8);
+    // channel rows padded to a multiple of 16 with zeros (matrix-core staging)
+    const size_t nchan_pad = round_up(nchan, 16);
+    st.d_Z.reserve(nchan_pad * nchunk * Zp * 8);
+    if (nchan_pad > nchan)
+        DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nchunk * Zp, 0,
+                              (nchan_pad - nchan) * nchunk * Zp * 8, stream));
     ctx.timer.begin(StageTimer::UNPACK, stream);
     DD_HIP(dd_tdd_transpose(d_in, nsamps, nchan, 8, nchan, st.d_rows.as<uint32_t>(), row_words, stream));
     ctx.timer.end(StageTimer::UNPACK, stream);
