@@ -64,6 +64,32 @@ class Profile(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class StreamStats(C.Structure):
+    _fields_ = [
+        ("pushes", C.c_ulong), ("mean_ms", C.c_double), ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double), ("max_ms", C.c_double), ("samples_per_s", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class FilHeader(C.Structure):
+    _fields_ = [
+        ("header_bytes", C.c_long), ("data_bytes", C.c_long),
+        ("nchans", C.c_int), ("nsamps", C.c_int), ("nbits", C.c_int), ("nifs", C.c_int),
+        ("machine_id", C.c_int), ("telescope_id", C.c_int), ("nbeams", C.c_int), ("ibeam", C.c_int),
+        ("tstart", C.c_double), ("tsamp", C.c_double), ("fch1", C.c_double), ("foff", C.c_double),
+        ("src_raj", C.c_double), ("src_dej", C.c_double), ("az_start", C.c_double), ("za_start", C.c_double),
+        ("source_name", C.c_char * 80),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["source_name"] = self.source_name.decode()
+        return d
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported entry point of include/*.h
@@ -111,6 +137,19 @@ _SIG = [
     ("dedisp_amd_get_implementation", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("dedisp_amd_generate_delay_table", C.c_int, [C.c_ulong, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     ("dedisp_amd_version", C.c_char_p, []),
+    ("dedisp_amd_stream_bytes", C.c_ulong, [C.c_void_p, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong]),
+    ("dedisp_amd_stream_create", C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_ulong, C.c_ulong,
+                                           C.c_ulong, C.c_ulong]),
+    ("dedisp_amd_stream_push", C.c_int, [C.c_void_p, C.c_ulong, C.c_void_p, C.c_void_p, C.c_uint]),
+    ("dedisp_amd_stream_reset", C.c_int, [C.c_void_p]),
+    ("dedisp_amd_stream_get_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
+    ("dedisp_amd_stream_history", C.c_ulong, [C.c_void_p]),
+    ("dedisp_amd_stream_destroy", None, [C.c_void_p]),
+    ("dedisp_amd_fil_read_header", C.c_int, [C.c_char_p, C.POINTER(FilHeader)]),
+    ("dedisp_amd_fil_read_data", C.c_int, [C.c_char_p, C.POINTER(FilHeader), C.c_void_p, C.c_ulong]),
+    ("dedisp_amd_fil_decimate", C.c_ulong, [C.c_void_p, C.c_ulong, C.c_ulong, C.c_ulong]),
+    ("dedisp_amd_write_dat", C.c_int, [C.c_char_p, C.c_float, C.c_void_p, C.c_ulong]),
+    ("dedisp_amd_write_inf", C.c_int, [C.c_char_p, C.POINTER(FilHeader), C.c_float, C.c_ulong]),
 ]
 EXPORTED = [s[0] for s in _SIG]
 
@@ -328,6 +367,65 @@ def pack_samples(samples: np.ndarray, nbits: int) -> np.ndarray:
     for k in range(per):
         out |= (v[:, :, k] & ((1 << nbits) - 1)) << (k * nbits)
     return out
+
+
+class Stream:
+    """Streaming front end over a plan (dedisp_amd_stream_*): nbeams beams that
+    deliver chunks of chunk_nsamps samples; each push returns chunk_nsamps
+    outputs per DM, delayed by the plan's max_delay (see dedisp_amd.h)."""
+
+    def __init__(self, plan: "Plan", nbeams: int, chunk_nsamps: int, in_nbits: int, out_nbits: int = 32):
+        h = C.c_void_p(0)
+        _check(lib().dedisp_amd_stream_create(C.byref(h), plan.handle, nbeams, chunk_nsamps, in_nbits, out_nbits),
+               "dedisp_amd_stream_create")
+        self._h = h
+        self.plan = plan
+        self.nbeams, self.chunk, self.in_nbits, self.out_nbits = nbeams, chunk_nsamps, in_nbits, out_nbits
+
+    @property
+    def history(self) -> int:
+        return int(lib().dedisp_amd_stream_history(self._h))
+
+    def push_raw(self, beam: int, chunk_ptr, out_ptr, flags: int = 0) -> int:
+        return lib().dedisp_amd_stream_push(self._h, beam, _ptr(chunk_ptr), _ptr(out_ptr), flags)
+
+    def push(self, beam: int, chunk: np.ndarray) -> np.ndarray:
+        """Host arrays: chunk uint8 [chunk_nsamps, row_bytes] -> [ndm, chunk_nsamps]."""
+        chunk = np.ascontiguousarray(chunk)
+        dt = {8: np.uint8, 16: np.uint16, 32: np.float32}[self.out_nbits]
+        out = np.empty((self.plan.dm_count, self.chunk), dtype=dt)
+        _check(self.push_raw(beam, chunk, out), "dedisp_amd_stream_push")
+        return out
+
+    def reset(self) -> None:
+        _check(lib().dedisp_amd_stream_reset(self._h), "dedisp_amd_stream_reset")
+
+    def stats(self) -> dict:
+        st = StreamStats()
+        _check(lib().dedisp_amd_stream_get_stats(self._h, C.byref(st)), "dedisp_amd_stream_get_stats")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dedisp_amd_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_filterbank(path) -> tuple[FilHeader, np.ndarray]:
+    """SIGPROC filterbank header and samples (uint8 [nsamps, nchans*nbits/8])."""
+    h = FilHeader()
+    p = str(path).encode()
+    _check(lib().dedisp_amd_fil_read_header(p, C.byref(h)), "dedisp_amd_fil_read_header")
+    row = h.nchans * h.nbits // 8
+    data = np.empty((h.nsamps, row), dtype=np.uint8)
+    _check(lib().dedisp_amd_fil_read_data(p, C.byref(h), _ptr(data), data.nbytes), "dedisp_amd_fil_read_data")
+    return h, data
 
 
 def unpack_samples(packed: np.ndarray, nbits: int, nchans: int) -> np.ndarray:

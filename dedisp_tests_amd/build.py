@@ -20,6 +20,8 @@ REPO = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libdedisp.so"
+CLI = PKG / "cli"
+BIN = PKG / "bin"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("DEDISP_OFFLOAD_ARCH", "gfx950")
 
@@ -66,6 +68,13 @@ def build(verbose: bool = True) -> Path:
           *map(str, objs), "-o", str(tmp), f"-L{ROCM / 'lib'}", "-lamdhip64",
           f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
+    # command-line front ends (host only, linked against the library)
+    BIN.mkdir(exist_ok=True)
+    for src in sorted(CLI.glob("*.cpp")):
+        exe = BIN / src.stem
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{REPO / 'include'}", str(src), "-o",
+              str(exe) + ".tmp", f"-L{PKG}", "-ldedisp", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(str(exe) + ".tmp", exe)
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", file=sys.stderr)
     return LIB

@@ -15,11 +15,6 @@
 
 #include "internal.hpp"
 
-struct dedisp_plan_struct {
-    std::unique_ptr<dedisp::Plan> ptr;
-    dedisp_implementation impl = DEDISP_TDD;
-};
-
 namespace {
 
 int g_device_idx = 0;                                  // dedisp.cpp:31

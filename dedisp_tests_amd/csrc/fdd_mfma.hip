@@ -120,14 +120,16 @@ __device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uin
     lo = __builtin_bit_cast(uint32_t, l);
 }
 
-// Round-to-nearest float16 pair (a in the low half). Written as the one
-// conversion instruction so the compiler does not fuse it into the producing
-// FMA (which then has to be computed twice).
+// Round-to-nearest float16 pair (a in the low half). The empty asm pins the
+// inputs in registers so the conversion stays one v_cvt_pk_f16_f32 instead of
+// being fused into the producing FMA (which then has to be computed twice);
+// the conversion itself is left to the compiler, whose hazard tracking covers
+// the MFMA that reads it.
 __device__ __forceinline__ uint32_t pack_f16x2(float a, float b)
 {
-    uint32_t r;
-    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
+    asm volatile("" : "+v"(a), "+v"(b));
+    const f16x2 v = {(_Float16)a, (_Float16)b};
+    return __builtin_bit_cast(uint32_t, v);
 }
 
 // e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used) is T_hi[m>>8] * T_lo[m&255];
@@ -169,6 +171,7 @@ void fdd_mfma_k(
 
     for (int t = tid; t < 512; t += NT)
         reinterpret_cast<float2*>(tab)[t] = tab65536[t];
+    __syncthreads();  // the table is read before the first K-step barrier
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
     // Every load is unconditional and in bounds (Z has its channel count

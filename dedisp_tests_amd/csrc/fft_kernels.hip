@@ -66,9 +66,16 @@ struct FftShape {
     static constexpr int T2 = A * C;  // pass-2 threads (each B values)
     static constexpr int T3 = B * C;  // pass-3 threads (each A values)
     static constexpr int T = (T1 > T2 ? (T1 > T3 ? T1 : T3) : (T2 > T3 ? T2 : T3));
-    // LDS floats of one exchange round (one component of every value, padded rows)
+    // LDS floats of one exchange round (one component of every value).
+    // Exchange 1 -> 2: rows of B values padded by one. Exchange 2 -> 3:
+    // groups of B*A values (one k2) padded by G2, chosen (by a bank
+    // simulation over 32 and 64 banks) so that both the pass-2 stores (lanes
+    // along n0) and the pass-3 loads (lanes along k2) are at most 2-way
+    // conflicted; unpadded rows of A+1 put every pass-3 lane of a half-wave
+    // in one bank.
     static constexpr int X1 = A * C * (B + 1);
-    static constexpr int X2 = B * C * (A + 1);
+    static constexpr int G2 = (A == 32) ? 1 : 9;
+    static constexpr int X2 = C * (B * A + G2);
     static constexpr int LDS_FLOATS = X1 > X2 ? X1 : X2;
 };
 
@@ -150,7 +157,7 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
         }
     }
 
-    // exchange 2 -> 3: element (n0, k1, k2) at (k2*B + k1)*(A+1) + n0
+    // exchange 2 -> 3: element (n0, k1, k2) at k2*(B*A + G2) + k1*A + n0
     float2 q[A];
     const int k2_3 = t % C, k1_3 = t / C;
     const bool act3 = t < S::T3;
@@ -160,13 +167,13 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
         if (act2) {
 #pragma unroll
             for (int k1 = 0; k1 < B; ++k1)
-                lds[(k2_2 * B + k1) * (A + 1) + n0_2] = comp ? u[k1].y : u[k1].x;
+                lds[k2_2 * (B * A + S::G2) + k1 * A + n0_2] = comp ? u[k1].y : u[k1].x;
         }
         __syncthreads();
         if (act3) {
 #pragma unroll
             for (int n0 = 0; n0 < A; ++n0) {
-                const float val = lds[(k2_3 * B + k1_3) * (A + 1) + n0];
+                const float val = lds[k2_3 * (B * A + S::G2) + k1_3 * A + n0];
                 if (comp) q[n0].y = val;
                 else q[n0].x = val;
             }

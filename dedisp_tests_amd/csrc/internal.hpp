@@ -106,3 +106,10 @@ void make_delay_table(dedisp_size nchans, dedisp_float dt, dedisp_float f0,
                       dedisp_float df, dedisp_float* out);
 
 }  // namespace dedisp
+
+// The object behind a C-API dedisp_plan handle.
+struct dedisp_plan_struct {
+    std::unique_ptr<dedisp::Plan> ptr;
+    dedisp_implementation impl = DEDISP_TDD;
+};
+

@@ -111,6 +111,9 @@ public:
     virtual void set_killmask(const bool_type* killmask) override;
 
     int get_device() const;
+    /* Device context (streams, device buffers) of the plan; opaque to
+     * callers, used by the streaming front end. */
+    GPUContext& context() const { return *m_ctx; }
 
     /* Stage timings of the last execute (extension, dedisp_amd.h). */
     void set_profiling(bool enable);

@@ -86,6 +86,85 @@ dedisp_error dedisp_amd_generate_delay_table(dedisp_size nchans, dedisp_float dt
 /* Library version string. */
 const char* dedisp_amd_version(void);
 
+/* ---------------------------------------------------------------------------
+ * Streaming front end (no reference counterpart: the caller-side loop a
+ * real-time pipeline runs around dedisp_execute).
+ *
+ * A stream serves nbeams independent beams that deliver fixed chunks of
+ * chunk_nsamps samples. Each beam keeps the last max_delay samples of its
+ * input on the device, so every push of one chunk yields chunk_nsamps output
+ * samples per DM: after the n-th push (n = 0, 1, ...) of a beam the outputs are
+ * the dedispersed samples t = n*chunk_nsamps - max_delay + i, i < chunk_nsamps,
+ * of the beam's concatenated input (samples before the first push read as 0).
+ * Outputs for t >= 0 equal those of one dedisp_execute over the whole series.
+ * The plan (TDD, legacy or FDD) must have its DM list set; an FDD plan is
+ * switched to time-series output. max_delay is fixed at creation.
+ * --------------------------------------------------------------------------- */
+typedef struct dedisp_amd_stream_struct* dedisp_amd_stream;
+
+typedef struct {
+    unsigned long pushes;       /* pushes since creation or the last reset */
+    double        mean_ms;      /* wall time per push, host-synchronous */
+    double        p50_ms;
+    double        p99_ms;
+    double        max_ms;
+    double        samples_per_s;  /* input samples (all beams) per second of push time */
+} dedisp_amd_stream_stats;
+
+/* Device bytes a stream with these parameters allocates (ring buffers and
+ * output staging), for admission against free HBM. */
+dedisp_size  dedisp_amd_stream_bytes(const dedisp_plan plan, dedisp_size nbeams,
+                                     dedisp_size chunk_nsamps, dedisp_size in_nbits,
+                                     dedisp_size out_nbits);
+/* Fails with DEDISP_MEM_ALLOC_FAILED when dedisp_amd_stream_bytes exceeds the
+ * device's free memory less a 10 % reserve (admission). */
+dedisp_error dedisp_amd_stream_create(dedisp_amd_stream* stream, dedisp_plan plan,
+                                      dedisp_size nbeams, dedisp_size chunk_nsamps,
+                                      dedisp_size in_nbits, dedisp_size out_nbits);
+/* chunk: chunk_nsamps time-major samples of the beam (nchans*in_nbits/8 bytes
+ * each); out: ndm rows of chunk_nsamps outputs (out_nbits/8 bytes each).
+ * flags: DEDISP_DEVICE_POINTERS when both are device pointers (else host). */
+dedisp_error dedisp_amd_stream_push(dedisp_amd_stream stream, dedisp_size beam,
+                                    const dedisp_byte* chunk, dedisp_byte* out,
+                                    unsigned flags);
+/* Zeroes every beam's history and the statistics. */
+dedisp_error dedisp_amd_stream_reset(dedisp_amd_stream stream);
+dedisp_error dedisp_amd_stream_get_stats(const dedisp_amd_stream stream,
+                                         dedisp_amd_stream_stats* stats);
+dedisp_size  dedisp_amd_stream_history(const dedisp_amd_stream stream);
+void         dedisp_amd_stream_destroy(dedisp_amd_stream stream);
+
+/* ---------------------------------------------------------------------------
+ * Filterbank front end: SIGPROC filterbank input, PRESTO .dat/.inf output
+ * (the reference's bin/fil/fil.hpp as library calls; host only, no device).
+ * File errors return DEDISP_UNKNOWN_ERROR.
+ * --------------------------------------------------------------------------- */
+typedef struct {
+    long   header_bytes;   /* bytes before the samples */
+    long   data_bytes;     /* bytes of samples after the header */
+    int    nchans, nsamps, nbits, nifs;  /* nsamps from the file size */
+    int    machine_id, telescope_id, nbeams, ibeam;
+    double tstart, tsamp, fch1, foff;
+    double src_raj, src_dej, az_start, za_start;
+    char   source_name[80];
+} dedisp_amd_fil_header;
+
+/* reference fil.hpp:288-350 (read_header) */
+dedisp_error dedisp_amd_fil_read_header(const char* path, dedisp_amd_fil_header* header);
+/* The first nbytes sample bytes (time-major, channels fastest). */
+dedisp_error dedisp_amd_fil_read_data(const char* path, const dedisp_amd_fil_header* header,
+                                      dedisp_byte* data, dedisp_size nbytes);
+/* reference fil.hpp:31-52: 8-bit samples averaged over groups of factor in
+ * place; returns the new sample count nsamps / factor. */
+dedisp_size  dedisp_amd_fil_decimate(dedisp_byte* data, dedisp_size nchans, dedisp_size nsamps,
+                                     dedisp_size factor);
+/* <prefix>_DM<dm %.3f>.dat: numout float samples (fil.hpp:262-281). */
+dedisp_error dedisp_amd_write_dat(const char* prefix, dedisp_float dm, const dedisp_float* series,
+                                  dedisp_size numout);
+/* <prefix>_DM<dm %.3f>.inf: PRESTO info file (fil.hpp:468-539). */
+dedisp_error dedisp_amd_write_inf(const char* prefix, const dedisp_amd_fil_header* header,
+                                  dedisp_float dm, dedisp_size numout);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,110 @@
+"""Streaming front end (dedisp_amd_stream_*) against one-shot execution.
+
+After the n-th push of a beam the stream returns the dedispersed samples
+t = n*chunk - max_delay + i of the beam's concatenated input. Every output with
+t >= 0 reads only pushed samples, so it must equal the one-shot dedisp_execute
+over the concatenation: bit-exact for TDD, within the FDD-FP32 tolerance for
+FDD.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def _stitch(outs, chunk, hist):
+    """Concatenated stream outputs for t >= 0 ([ndm, total - hist])."""
+    full = np.concatenate(outs, axis=1)  # t = -hist .. n*chunk - hist - 1
+    return full[:, hist:]
+
+
+def test_tdd_stream_matches_one_shot(dd):
+    from dedisp_tests_amd import synth
+    nch, dt, f0, df = 256, 64e-6, 1581.8, -0.390625
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.TDD)
+    plan.set_dm_list(np.arange(0, 200, 2, dtype=f32))
+    hist = plan.max_delay
+    chunk, npush = 1500, 6  # chunk < max_delay exercises a history longer than a chunk
+    assert chunk < hist
+    rng = np.random.default_rng(5)
+    series = [rng.integers(0, 16, size=(chunk * npush, nch), dtype=np.uint32) for _ in range(2)]
+    packed = [dd.pack_samples(s, 4) for s in series]
+    st = dd.Stream(plan, nbeams=2, chunk_nsamps=chunk, in_nbits=4, out_nbits=32)
+    assert st.history == hist
+    outs = [[], []]
+    for n in range(npush):  # interleave the beams: histories must stay apart
+        for b in (1, 0):
+            outs[b].append(st.push(b, packed[b][n * chunk:(n + 1) * chunk]))
+    for b in range(2):
+        got = _stitch(outs[b], chunk, hist)
+        ref = plan.execute(packed[b], 4, 32)
+        assert got.shape == ref.shape
+        assert np.array_equal(got, ref)
+    s = st.stats()
+    assert s["pushes"] == 2 * npush and s["p99_ms"] >= s["p50_ms"] > 0
+    # reset: the history is zero again, so the first push equals a fresh stream
+    st.reset()
+    again = st.push(0, packed[0][:chunk])
+    assert np.array_equal(again, outs[0][0])
+    st.close()
+
+
+def test_tdd_stream_8bit_out_and_device_pointers(dd):
+    import torch
+    nch = 128
+    plan = dd.Plan(nch, 64e-6, 1581.8, -0.78125, impl=dd.TDD)
+    plan.set_dm_list(np.linspace(0, 150, 33, dtype=f32))
+    hist, chunk, npush = plan.max_delay, 4096, 3
+    rng = np.random.default_rng(9)
+    data = rng.integers(0, 256, size=(chunk * npush, nch), dtype=np.uint8)
+    st = dd.Stream(plan, nbeams=1, chunk_nsamps=chunk, in_nbits=8, out_nbits=8)
+    d_out = torch.empty((plan.dm_count, chunk), dtype=torch.uint8, device="cuda")
+    outs = []
+    for n in range(npush):
+        d_in = torch.from_numpy(np.ascontiguousarray(data[n * chunk:(n + 1) * chunk])).cuda()
+        torch.cuda.synchronize()
+        assert st.push_raw(0, d_in, d_out, dd.DEVICE_POINTERS) == 0
+        outs.append(d_out.cpu().numpy().copy())
+    got = _stitch(outs, chunk, hist)
+    ref = plan.execute(data, 8, 8)
+    assert np.array_equal(got, ref)
+
+
+def test_fdd_stream_matches_oracle(dd, orc):
+    nch, dt, f0, df = 64, 64e-6, 1581.8, -2.0
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    dms = np.linspace(0, 60, 9, dtype=f32)
+    plan.set_dm_list(dms)
+    hist, chunk, npush = plan.max_delay, 8000, 4
+    rng = np.random.default_rng(3)
+    data = np.clip(np.rint(rng.normal(128, 20, size=(chunk * npush, nch))), 0, 255).astype(np.uint8)
+    st = dd.Stream(plan, nbeams=1, chunk_nsamps=chunk, in_nbits=8, out_nbits=32)
+    outs = [st.push(0, data[n * chunk:(n + 1) * chunk]) for n in range(npush)]
+    got = _stitch(outs, chunk, hist)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    ref = orc.fdd_timeseries(nch, delays, dms, chunk * npush, hist, data)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-4, err
+
+
+def test_stream_admission_and_errors(dd):
+    plan = dd.Plan(64, 64e-6, 1581.8, -2.0, impl=dd.TDD)
+    import ctypes as C
+    h = C.c_void_p(0)
+    # no DM list yet
+    assert dd.lib().dedisp_amd_stream_create(C.byref(h), plan.handle, 1, 1000, 8, 32) == dd.DEDISP_NO_DM_LIST_SET
+    plan.set_dm_list(np.array([0.0, 10.0], f32))
+    # far beyond any device's memory: refused by the admission check
+    need = dd.lib().dedisp_amd_stream_bytes(plan.handle, 1 << 20, 1 << 20, 8, 32)
+    assert need > (1 << 40)
+    assert dd.lib().dedisp_amd_stream_create(C.byref(h), plan.handle, 1 << 20, 1 << 20, 8, 32) == \
+        dd.DEDISP_MEM_ALLOC_FAILED
+    assert dd.lib().dedisp_amd_stream_create(C.byref(h), plan.handle, 1, 1000, 3, 32) == \
+        dd.DEDISP_UNSUPPORTED_IN_NBITS
+    st = dd.Stream(plan, nbeams=2, chunk_nsamps=1000, in_nbits=8)
+    buf = np.zeros((1000, 64), np.uint8)
+    out = np.zeros((2, 1000), np.float32)
+    assert st.push_raw(2, buf, out) == dd.DEDISP_INVALID_POINTER  # beam out of range
+    plan.set_dm_list(np.array([0.0, 20.0], f32))                  # max_delay changes
+    assert st.push_raw(0, buf, out) == dd.DEDISP_INVALID_PLAN
