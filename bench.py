@@ -20,6 +20,8 @@ Prints one JSON line. Besides the contract fields it carries:
   valu_roofline  the integer-add view of the same kernel (channel-adds / s
                  against the int32 VALU lane rate; the kernel packs 4 sums per
                  32-bit lane, so the fraction can exceed 1)
+  stream         the streaming front end on 32 beams x 1 s chunks, 1024 ch,
+                 3000 DMs: push latency percentiles and real-time factor
   cpu_baseline   the reference's CPU FDD plan (restated as written, OpenMP)
                  timed on this host on a bounded DM sample, scaled to the
                  full DM list
@@ -37,6 +39,21 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "tdd-1024ch-2p20x4bit-1167dm_hbm-traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the channel-sum kernel, from the committed
+    FETCH_SIZE / WRITE_SIZE counter summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 VALU_INT_PEAK_T = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops / s (256 CU x 64 lanes x 2.4 GHz)
 
 NCH, DT, F0, DF = 1024, 54.6e-6, 1530.0, -0.3125
@@ -110,7 +127,7 @@ def cpu_baseline(seconds_budget: float = 30.0):
 MFMA_PEAK_T = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # dense TFLOP/s
 
 
-def fdd_leg(torch, dd, steps: int, warmup: int) -> dict:
+def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16")) -> dict:
     """FDD on 1024 ch x 2^20 8-bit samples (N(128,20), five pulses), 2048 linear DM
     trials 0..1023.5, overlap-save chunks of 2^16 samples, time-series output,
     input and output in HBM. One line per contraction form with its error against
@@ -145,6 +162,8 @@ def fdd_leg(torch, dd, steps: int, warmup: int) -> dict:
                                 dms[sample_rows], NSAMPS, 0, host_in)
     out = {"workload": "fdd-1024ch-2^20x8bit-2048dm-nfft65536", "ndm": 2048, "nfft": 65536}
     for name, prec in (("fp32", dd.FDD_FP32), ("bf16", dd.FDD_BF16), ("fp16", dd.FDD_FP16)):
+        if name not in forms:
+            continue
         plan = dd.Plan(NCH, DT, F0, DF, impl=dd.FDD)
         plan.fdd_set_output(dd.FDD_TIMESERIES)
         plan.fdd_set_segmented(True, 65536)
@@ -185,9 +204,52 @@ def fdd_leg(torch, dd, steps: int, warmup: int) -> dict:
         }
         del d_out, plan
         torch.cuda.empty_cache()
-    out["speedup_bf16_vs_fp32"] = out["bf16"]["value"] / out["fp32"]["value"]
-    out["speedup_fp16_vs_fp32"] = out["fp16"]["value"] / out["fp32"]["value"]
+    if "fp32" in out and "bf16" in out:
+        out["speedup_bf16_vs_fp32"] = out["bf16"]["value"] / out["fp32"]["value"]
+    if "fp32" in out and "fp16" in out:
+        out["speedup_fp16_vs_fp32"] = out["fp16"]["value"] / out["fp32"]["value"]
     return out
+
+
+def stream_leg(torch, dd, rounds: int = 3) -> dict:
+    """Streaming front end: 32 beams delivering 1 s chunks (15625 samples at
+    64 us) of 1024 channels from 1530 MHz down in 0.3125 MHz steps, 8-bit
+    N(128, 20), 3000 linear DM trials 0..1500, TDD plan, float
+    output. Round 0 fills the histories and is not timed; every later round
+    pushes one chunk per beam with input and output in HBM."""
+    nb, chunk, nch, dt = 32, 15625, 1024, 64e-6
+    plan = dd.Plan(nch, dt, 1530.0, -0.3125, impl=dd.TDD)
+    plan.set_dm_list(np.linspace(0.0, 1500.0, 3000, dtype=np.float32))
+    st = dd.Stream(plan, nb, chunk, 8, 32)
+    d_out = torch.empty((plan.dm_count, chunk), dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    round_s = []
+    for r in range(rounds):
+        x = torch.randn((nb, chunk, nch), generator=g, device="cuda", dtype=torch.float32)
+        chunks = torch.clamp(torch.round(x * 20.0 + 128.0), 0, 255).to(torch.uint8)
+        del x
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for b in range(nb):
+            rc = st.push_raw(b, chunks[b], d_out, dd.DEVICE_POINTERS)
+            if rc != 0:
+                raise dd.DedispError(rc, "dedisp_amd_stream_push")
+        if r > 0:
+            round_s.append(time.perf_counter() - t0)
+        del chunks
+    stats = st.stats()
+    st.close()
+    t_round = float(np.median(round_s))
+    return {
+        "workload": "stream-32beam-1024ch-15625x8bit-3000dm", "beams": nb, "chunk_nsamps": chunk,
+        "max_delay": plan.max_delay, "ndm": plan.dm_count, "timed_rounds": len(round_s),
+        "round_ms": 1e3 * t_round,
+        "real_time_factor": nb * chunk * dt / t_round,
+        "dm_samples_per_s": plan.dm_count * chunk * nb / t_round,
+        "push_ms": {"p50": stats["p50_ms"], "p99": stats["p99_ms"], "mean": stats["mean_ms"],
+                    "max": stats["max_ms"], "pushes": stats["pushes"], "note": "all rounds, host-synchronous"},
+    }
 
 
 def main():
@@ -197,6 +259,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fdd", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--fdd-forms", default="fp32,bf16,fp16",
+                    help="contraction forms of the FDD leg (counter runs use one)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device to device twice (the known-size kernel that "
+                         "scales FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -205,6 +273,15 @@ def main():
 
     import torch
     import dedisp_tests_amd as dd
+
+    if args.calibrate_copy:
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes": x.numel() * 4, "copies": 2}))
+        return
 
     s, d_in = make_input(torch)
     plan = dd.Plan(NCH, DT, F0, DF, impl=dd.TDD)
@@ -267,7 +344,8 @@ def main():
                    "in_nbits": NBITS, "out_nbits": 32, "ndm": ndm, "max_delay": maxd,
                    "dm_list": "dedisp_generate_dm_list(0,1000,40us,1.25)"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/tdd-1024ch-2p20x4bit-1167dm_hbm-traffic.md",
                      "kernel": "tdd_lds_k", "kernel_ms": k_ms,
                      "algorithmic_bytes": in_bytes + out_bytes},
         "valu_roofline": {"bound": "valu", "achieved": adds_t, "peak": VALU_INT_PEAK_T,
@@ -279,7 +357,12 @@ def main():
     del d_out, d_in, s
     torch.cuda.empty_cache()
     if not args.no_fdd:
-        line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1)
+        line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1 if args.warmup else 0,
+                              forms=args.fdd_forms.split(","))
+        torch.cuda.empty_cache()
+    if not args.no_stream:
+        line["stream"] = stream_leg(torch, dd)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

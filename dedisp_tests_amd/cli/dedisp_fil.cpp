@@ -116,8 +116,8 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "dedisp_fil: %d polarisations (1 supported)\n", h.nifs);
         return 1;
     }
-    if (h.nbits != 8 && (impl == DEDISP_FDD || ndec > 1 ||
-                         (h.nbits != 1 && h.nbits != 2 && h.nbits != 4 && h.nbits != 16))) {
+    if (h.nbits != 8 && (ndec > 1 || (h.nbits != 1 && h.nbits != 2 && h.nbits != 4 &&
+                                      !(h.nbits == 16 && impl != DEDISP_FDD)))) {
         std::fprintf(stderr, "dedisp_fil: %d-bit samples are not supported here\n", h.nbits);
         return 1;
     }

@@ -10,30 +10,10 @@
 
 namespace {
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// X[k] of the real series from the packed transform Z (M points):
-// X = (Z[k] + conj Z[M-k])/2 - i/2 e^{-2 pi i k/nfft} (Z[k] - conj Z[M-k]).
-__device__ __forceinline__ float2 unpack_bin(const float2* __restrict__ zrow, uint32_t k, uint32_t M,
-                                             float2 wk /* e^{-2 pi i k/nfft} */)
-{
-    const float2 a = zrow[k & (M - 1)];
-    const float2 bz = zrow[(M - k) & (M - 1)];
-    const float2 b = make_float2(bz.x, -bz.y);
-    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
-    const float2 d = make_float2(a.x - b.x, a.y - b.y);
-    const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // d / (2i)
-    const float2 r = cmul(o, wk);
-    return make_float2(e.x + r.x, e.y + r.y);
-}
-
 // Lane = bin, wave = DPW DMs, CG chunks per pass over the channels.
 template <int DPW, int CG>
 __global__ __launch_bounds__(256) void fdd_contract_fp32_k(
-    const float2* __restrict__ Z, size_t Zp, const int32_t* __restrict__ delays, uint32_t ndm,
+    const float2* __restrict__ X, size_t Xp, const int32_t* __restrict__ delays, uint32_t ndm,
     uint32_t nchan, uint32_t nchunk, const float2* __restrict__ ph, uint32_t nfft,
     float2* __restrict__ Y, size_t Yp)
 {
@@ -42,8 +22,6 @@ __global__ __launch_bounds__(256) void fdd_contract_fp32_k(
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t dm0 = blockIdx.y * (4 * DPW) + wave * DPW;
     if (k > M) return;
-    const float2 pk = ph[k];
-    const float2 wk = make_float2(pk.x, -pk.y);
     uint32_t drow[DPW];
 #pragma unroll
     for (int i = 0; i < DPW; ++i) drow[i] = min(dm0 + i, ndm - 1) * nchan;
@@ -59,7 +37,7 @@ __global__ __launch_bounds__(256) void fdd_contract_fp32_k(
 #pragma unroll
             for (int j = 0; j < CG; ++j) {
                 const uint32_t ch = min(g0 + j, nchunk - 1);
-                x[j] = unpack_bin(Z + ((size_t)c * nchunk + ch) * Zp, k, M, wk);
+                x[j] = X[((size_t)c * nchunk + ch) * Xp + k];
             }
 #pragma unroll
             for (int i = 0; i < DPW; ++i) {
@@ -84,14 +62,14 @@ __global__ __launch_bounds__(256) void fdd_contract_fp32_k(
 
 }  // namespace
 
-extern "C" hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delays, size_t ndm,
+extern "C" hipError_t dd_fdd_contract_fp32(const float2* X, size_t Xp, const int32_t* delays, size_t ndm,
                                            size_t nchan, size_t nchunk, const float2* ph, size_t nfft,
                                            float2* Y, size_t Yp, hipStream_t stream)
 {
     constexpr int DPW = 4, CG = 8;
     const size_t M = nfft / 2;
     dim3 grid((unsigned)((M + 1 + 63) / 64), (unsigned)((ndm + 4 * DPW - 1) / (4 * DPW)));
-    hipLaunchKernelGGL((fdd_contract_fp32_k<DPW, CG>), grid, dim3(256), 0, stream, Z, Zp, delays,
+    hipLaunchKernelGGL((fdd_contract_fp32_k<DPW, CG>), grid, dim3(256), 0, stream, X, Xp, delays,
                        (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, ph, (uint32_t)nfft, Y, Yp);
     return hipGetLastError();
 }

@@ -12,16 +12,18 @@ extern "C" {
 // nfft values the on-chip transform engine handles.
 int dd_fdd_fft_supported(size_t nfft);
 
-// Forward transforms of every (channel, chunk): rows are channel-major 8-bit
-// samples (row_bytes per channel), chunk i starts at sample i*chunk_step and
-// spans nfft samples; x = (u8 - 127.5) * chan_scale[c] for sample t < nvalid and
-// 0 beyond. period > 0 reads the series periodically (sample t mod period; the
-// chunks then span at most two periods). Z: [nchan][nchunk][Zp] complex,
-// M = nfft/2 points of the packed transform. ph: [nfft] e^{+2 pi i m/nfft}.
+// Forward transforms of every (channel, chunk): rows are channel-major
+// nbits-bit samples (1, 2, 4 or 8; packed LSB-first along time in 32-bit
+// words, row_bytes per channel), chunk i starts at sample i*chunk_step and
+// spans nfft samples; x = (u - (2^nbits-1)/2) * chan_scale[c] for sample
+// t < nvalid and 0 beyond. period > 0 reads the series periodically (sample
+// t mod period; the chunks then span at most two periods). X:
+// [nchan][nchunk][Xp] complex, the real spectrum bins 0..nfft/2
+// (Xp >= nfft/2 + 1). ph: [nfft] e^{+2 pi i m/nfft}.
 hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step,
                           size_t nvalid, size_t period, const float* chan_scale,
-                          const float2* ph, size_t nfft, float2* Z, size_t nchan,
-                          size_t nchunk, size_t Zp, hipStream_t stream);
+                          const float2* ph, size_t nfft, float2* X, size_t nchan,
+                          size_t nchunk, size_t Xp, unsigned nbits, hipStream_t stream);
 
 // Inverse transforms of every (row, chunk) of Y ([nrows][nchunk][Yp], bins
 // 0..nfft/2): writes samples t < good of each chunk, scaled by 1/nfft, to
@@ -41,8 +43,8 @@ hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t nrows, size_
 
 // Phasor contraction, float32 on the vector ALUs:
 //   Y[i][ch][k] = sum_c e^{+2 pi i ((k*d[i][c]) mod nfft)/nfft} X_c[ch][k],  k = 0..nfft/2
-// with X_c formed from the packed spectra Z. delays: [ndm][nchan] int32.
-hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delays, size_t ndm,
+// with X the spectra of dd_fdd_forward. delays: [ndm][nchan] int32.
+hipError_t dd_fdd_contract_fp32(const float2* X, size_t Xp, const int32_t* delays, size_t ndm,
                                 size_t nchan, size_t nchunk, const float2* ph, size_t nfft,
                                 float2* Y, size_t Yp, hipStream_t stream);
 
@@ -55,10 +57,10 @@ hipError_t dd_fdd_contract_fp32(const float2* Z, size_t Zp, const int32_t* delay
 // tab65536 holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
 // (b < 256). The spectra are multiplied by xscale before the MFMA inputs are
 // formed and Y by 1/xscale after (a power of two keeping float16 in range).
-// Z must hold nchan rounded up to 16 channel rows, the extra rows zero.
-hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const uint16_t* q16,
+// X must hold nchan rounded up to 16 channel rows, the extra rows zero.
+hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint16_t* q16,
                                 size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
-                                const float2* tab65536, const float2* ph, size_t nfft,
+                                const float2* tab65536, size_t nfft,
                                 float xscale, float2* Y, size_t Yp, hipStream_t stream);
 
 #ifdef __cplusplus

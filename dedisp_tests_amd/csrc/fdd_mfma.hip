@@ -18,7 +18,7 @@
 // Spectra (B operand): X_k[c][chunk] for the K-step, staged in LDS in the
 //   planes of the form (float32: re, im, -im; bfloat16: re hi, re lo, im hi,
 //   im lo and the negated im planes; float16: re, im, -im), formed on load from
-//   the packed forward transform (real-input pairing). The next K-step's
+//   the real spectra of the forward transform. The next K-step's
 //   spectra and delays are loaded into registers while the current one is
 //   multiplied; one barrier per K-step (two LDS buffers).
 // Complex MAC as real MFMAs: Yr += Pr Xr + Pi (-Xi), Yi += Pr Xi + Pi Xr.
@@ -81,17 +81,6 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// h*X[k] of the real series from the packed transform Z (M points):
-// X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]), W^k = e^{-2 pi i k/nfft};
-// hw = h/2 * W^k, hh = h/2 (h = 0 gives exactly 0 for finite Z).
-__device__ __forceinline__ float2 pair_bin(float2 a, float2 bz, float2 hw, float hh)
-{
-    const float ux = a.x + bz.x, uy = a.y - bz.y;  // Z[k] + conj Z[M-k]
-    const float vx = a.x - bz.x, vy = a.y + bz.y;  // Z[k] - conj Z[M-k]
-    const float tx = vx * hw.x - vy * hw.y, ty = vx * hw.y + vy * hw.x;
-    return make_float2(fmaf(hh, ux, ty), fmaf(hh, uy, -tx));
-}
-
 __device__ __forceinline__ uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
 
 // A 32-bit constant in a scalar register, out of the compiler's reach. The
@@ -142,9 +131,9 @@ __global__ __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu(2, 2)))
 #endif
 void fdd_mfma_k(
-    const float2* __restrict__ Z, size_t Zp, const uint16_t* __restrict__ q16, size_t qpitch,
+    const float2* __restrict__ X, size_t Xp, const uint16_t* __restrict__ q16, size_t qpitch,
     uint32_t ndm, uint32_t nchan, uint32_t nchunk, const float2* __restrict__ tab65536,
-    const float2* __restrict__ ph, uint32_t nfft, float xscale, float yscale,
+    uint32_t nfft, float xscale, float yscale,
     float2* __restrict__ Y, size_t Yp, uint32_t nbb, uint32_t ndb)
 {
     using F = Form<PREC>;
@@ -174,32 +163,24 @@ void fdd_mfma_k(
     __syncthreads();  // the table is read before the first K-step barrier
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
-    // Every load is unconditional and in bounds (Z has its channel count
+    // Every load is unconditional and in bounds (X has its channel count
     // padded to a multiple of 16 with zero rows; out-of-range bins and chunks
-    // read a valid row and are multiplied by 0 in the pairing), so the
-    // compiler has no reason to wait for the loads before they are consumed
-    // at the end of the K-step.
+    // read a valid element and are multiplied by 0), so the compiler has no
+    // reason to wait for the loads before they are consumed at the end of the
+    // K-step.
     const int sb = tid & 3, sj = (tid >> 2) & 31, sg = (tid >> 7) & 1, sh0 = (tid >> 8) * HG;
     const uint32_t sk = k0 + sb, sch = ch0 + sj;
-    const float hh = (sk <= M && sch < nchunk) ? 0.5f * xscale : 0.f;
-    const float2 pk = ph[sk & (nfft - 1)];
-    const float2 hw = make_float2(hh * pk.x, -hh * pk.y);  // h/2 e^{-2 pi i k/nfft}
+    const float xs_scale = (sk <= M && sch < nchunk) ? xscale : 0.f;
     const int s_off = ((sb * 2 + sg) * NJ + sj) * 8 + 4 * sh0;  // element offset in a plane
-    // rows of channels 8g + 4h + q of the K-step: zlo/zhi point at bins k and
-    // M-k of channel 8g + 4*h0; channel 4u + q further is (4u + q)*rs on, a
-    // K-step 16*rs
-    const size_t rs = (size_t)nchunk * Zp;
-    const float2* zrow = Z + ((size_t)(8 * sg + 4 * sh0) * nchunk + min(sch, nchunk - 1)) * Zp;
-    const float2* zlo = zrow + (sk & (M - 1));
-    const float2* zhi = zrow + ((M - sk) & (M - 1));
+    // element (channel 8g + 4h0 + q, chunk, bin) of the K-step: xsrc + q*rs,
+    // a K-step further 16*rs
+    const size_t rs = (size_t)nchunk * Xp;
+    const float2* xsrc = X + ((size_t)(8 * sg + 4 * sh0) * nchunk + min(sch, nchunk - 1)) * Xp + min(sk, M);
 
-    float2 za[4 * HG], zb[4 * HG];
-    auto load_z = [&]() {
+    float2 xv[4 * HG];
+    auto load_x = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4 * HG; ++q) {
-            za[q] = zlo[q * rs];
-            zb[q] = zhi[q * rs];
-        }
+        for (int q = 0; q < 4 * HG; ++q) xv[q] = xsrc[q * rs];
     };
     auto store_x = [&](char* buf) {
 #pragma unroll
@@ -207,9 +188,8 @@ void fdd_mfma_k(
         float xr[4], xi[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 x = pair_bin(za[4 * u + q], zb[4 * u + q], hw, hh);
-            xr[q] = x.x;
-            xi[q] = x.y;
+            xr[q] = xv[4 * u + q].x * xs_scale;
+            xi[q] = xv[4 * u + q].y * xs_scale;
         }
         const int off = s_off + 4 * u;
         if constexpr (PREC == 0) {
@@ -280,11 +260,11 @@ void fdd_mfma_k(
 
     const uint32_t nks = (nchan + KS - 1) / KS;
     const size_t kstep = KS * rs;
-    load_z();
+    load_x();
     u32x4 qv = *reinterpret_cast<const u32x4*>(qrow);
     store_x(xs);
     if (late) lookups(qv);
-    if (nks > 1) zlo += kstep, zhi += kstep;
+    if (nks > 1) xsrc += kstep;
     __syncthreads();
 
     for (uint32_t ks = 0; ks < nks; ++ks) {
@@ -292,11 +272,9 @@ void fdd_mfma_k(
         const bool more = ks + 1 < nks;
         // prefetch K-step min(ks + 1, nks - 1): the last iteration re-reads
         // its own K-step (in bounds, unused)
-        load_z();
+        load_x();
         const u32x4 qn = *reinterpret_cast<const u32x4*>(qrow + min(ks + 1, nks - 1) * KS);
-        const size_t inc = ks + 2 < nks ? kstep : 0;
-        zlo += inc;
-        zhi += inc;
+        xsrc += ks + 2 < nks ? kstep : 0;
 
         if (!late) lookups(qv);
 
@@ -411,15 +389,15 @@ void fdd_mfma_k(
 
 }  // namespace
 
-extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp, const uint16_t* q16,
+extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint16_t* q16,
                                            size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
-                                           const float2* tab65536, const float2* ph, size_t nfft,
+                                           const float2* tab65536, size_t nfft,
                                            float xscale, float2* Y, size_t Yp, hipStream_t stream)
 {
     if (ndm == 0 || nchunk == 0) return hipSuccess;
     // layout preconditions of the vector loads and stores
     if (qpitch % 16 || qpitch < (nchan + 15) / 16 * 16 || Yp % 2 || nfft < 2 * NB || nfft > 65536 ||
-        (nfft & (nfft - 1)))
+        (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
         return hipErrorInvalidValue;
     const size_t M = nfft / 2;
     const uint32_t nbb = (uint32_t)((M + 1 + NB - 1) / NB);
@@ -429,8 +407,8 @@ extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* Z, size_t Zp,
     if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;
     const float yscale = 1.0f / xscale;
 #define DD_MFMA(P)                                                                                  \
-    hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, Z, Zp, q16,     \
-                       qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536, ph,      \
+    hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q16,     \
+                       qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536,          \
                        (uint32_t)nfft, xscale, yscale, Y, Yp, nbb, ndb)
     switch (prec) {
     case 0: DD_MFMA(0); break;

@@ -34,7 +34,7 @@ struct FDDState {
 
     DeviceBuffer d_in;      // staged 8-bit input (host-pointer mode)
     DeviceBuffer d_rows;    // channel-major 8-bit rows
-    DeviceBuffer d_Z;       // [nchan][nchunk][Zp] packed spectra
+    DeviceBuffer d_Z;       // [nchan padded to 16][nchunk][Xp] spectra, bins 0..nfft/2
     DeviceBuffer d_Y;       // [dm batch][nchunk][Yp] dedispersed spectra
     DeviceBuffer d_out;     // staged output (host-pointer mode)
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
@@ -120,14 +120,16 @@ void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbi
     if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
     if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
         DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
-    // 8-bit input, float output (src/fdd/FDDGPUPlan.cpp:66-67)
-    if (in_nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    // 8-bit input, float output (src/fdd/FDDGPUPlan.cpp:66-67); 1, 2 and 4-bit
+    // input are an extension (same centring, (u - (2^nbits-1)/2) / nchan)
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if ((m_nchans * in_nbits) % 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
     if (out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
     if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
 
     const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");
     const bool fourier = (m_state->output == DEDISP_AMD_FDD_FOURIER) && !segmented;
-    execute_fdd(nsamps, in, out, flags, fourier);
+    execute_fdd(nsamps, in, in_nbits, out, flags, fourier);
 }
 
 // Time-series output: overlap-save over the input, outputs t < nsamps - max_delay.
@@ -137,7 +139,7 @@ void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbi
 // (x zero beyond nsamps). s is produced by the same overlap-save path over the
 // N-periodic extension of the input (t = 0 .. N-1), then transformed per DM;
 // bins 0..N/2-1 are written as N floats (re, im).
-void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* out,
+void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in_nbits, byte_type* out,
                              unsigned flags, bool fourier)
 {
     GPUContext& ctx = *m_ctx;
@@ -148,6 +150,7 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
     ctx.profile = dedisp_amd_profile{};
     const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
     const size_t nchan = m_nchans, ndm = m_dm_count;
+    const size_t in_row = nchan * in_nbits / 8;  // bytes per input sample
     const size_t Nf = round_up(nsamps + m_max_delay, 16384);   // whole-series length
     const size_t nsc = fourier ? Nf : nsamps - m_max_delay;    // outputs per DM (t)
     const size_t out_floats = fourier ? Nf : nsc;              // floats per output row
@@ -200,7 +203,7 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
     const size_t good = (nfft - dmax) & ~size_t(1);
     const size_t nchunk = div_up(nsc, good);
     const size_t M = nfft / 2;
-    const size_t Zp = M, Yp = M + 64;
+    const size_t Xp = M + 64, Yp = M + 64;  // bins 0..M, rows 512-byte aligned
 
     if (st.ph_nfft != nfft) {
         std::vector<float> ph(2 * nfft);
@@ -256,9 +259,9 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
     if (device_ptrs) {
         d_in = in;
     } else {
-        st.d_in.reserve(nsamps * nchan + 64);
+        st.d_in.reserve(nsamps * in_row + 64);
         ctx.timer.begin(StageTimer::H2D, stream);
-        DD_HIP(hipMemcpyAsync(st.d_in.ptr, in, nsamps * nchan, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_in.ptr, in, nsamps * in_row, hipMemcpyHostToDevice, stream));
         ctx.timer.end(StageTimer::H2D, stream);
         d_in = st.d_in.as<uint8_t>();
     }
@@ -291,21 +294,22 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
 
     // ---- transpose to channel-major rows, forward transforms ----
     const size_t in_span = fourier ? nsamps : std::max(nsamps, (nchunk - 1) * good + nfft);
-    const size_t row_words = div_up(in_span, 4 * 64) * 64;
+    const size_t row_words = div_up(in_span, (32 / in_nbits) * 64) * 64;
     st.d_rows.reserve(nchan * row_words * 4);
     // channel rows padded to a multiple of 16 with zeros (matrix-core staging)
     const size_t nchan_pad = round_up(nchan, 16);
-    st.d_Z.reserve(nchan_pad * nchunk * Zp * 8);
+    st.d_Z.reserve(nchan_pad * nchunk * Xp * 8);
     if (nchan_pad > nchan)
-        DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nchunk * Zp, 0,
-                              (nchan_pad - nchan) * nchunk * Zp * 8, stream));
+        DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nchunk * Xp, 0,
+                              (nchan_pad - nchan) * nchunk * Xp * 8, stream));
     ctx.timer.begin(StageTimer::UNPACK, stream);
-    DD_HIP(dd_tdd_transpose(d_in, nsamps, nchan, 8, nchan, st.d_rows.as<uint32_t>(), row_words, stream));
+    DD_HIP(dd_tdd_transpose(d_in, nsamps, in_row, (unsigned)in_nbits, nchan, st.d_rows.as<uint32_t>(),
+                            row_words, stream));
     ctx.timer.end(StageTimer::UNPACK, stream);
     ctx.timer.begin(StageTimer::FFT, stream);
     DD_HIP(dd_fdd_forward(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, fourier ? Nf : 0,
-                          st.d_scale.as<float>(), ph, nfft, st.d_Z.as<float2>(), nchan, nchunk, Zp,
-                          stream));
+                          st.d_scale.as<float>(), ph, nfft, st.d_Z.as<float2>(), nchan, nchunk, Xp,
+                          (unsigned)in_nbits, stream));
     ctx.timer.end(StageTimer::FFT, stream);
 
     // ---- per DM batch: contraction, inverse transforms ----
@@ -330,12 +334,12 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, byte_type* o
         ctx.timer.begin(StageTimer::DEDISPERSE, stream);
         const int32_t* dl = st.d_delays.as<int32_t>() + b0 * nchan;
         if (valu)
-            DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Zp, dl, nb, nchan, nchunk, ph, nfft,
+            DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, dl, nb, nchan, nchunk, ph, nfft,
                                         st.d_Y.as<float2>(), Yp, stream));
         else
-            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Zp,
+            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Xp,
                                         st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan,
-                                        nchunk, st.d_tab.as<float2>(), ph, nfft, xscale,
+                                        nchunk, st.d_tab.as<float2>(), nfft, xscale,
                                         st.d_Y.as<float2>(), Yp, stream));
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
         ctx.timer.begin(StageTimer::IFFT, stream);

@@ -13,10 +13,10 @@
 //   pass 3: DFT_A over n0 for each (k1,k2)
 //
 // Real transforms of length nfft = 2M use the half-length packing
-// z[m] = x[2m] + i x[2m+1]; the forward kernel writes Z (M points) and the
-// consumer forms X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]),
-// W = e^{-2 pi i/nfft}. The inverse kernel forms its packed input from the
-// M+1 bins of each DM row at load time.
+// z[m] = x[2m] + i x[2m+1]; the forward kernel pairs its packed result Z into
+// X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]),
+// W = e^{-2 pi i/nfft}, and writes bins 0..M. The inverse kernel forms its
+// packed input from the M+1 bins of each DM row at load time.
 //
 // Twiddles come from one table ph[m] = e^{+2 pi i m / nfft}, m < nfft, built in
 // double precision on the host: W_M^j = conj(ph[2j]).
@@ -91,13 +91,21 @@ __device__ __forceinline__ float2 table_tw(const float2* __restrict__ ph, uint32
 }
 
 // The transform engine. load(n) gives input point n (pass-1 threads call it
-// for n = t + A*B*n2); store(k, value) receives output point k. ph is the
+// for n = t + A*B*n2); prep(v, active) then sees the C loaded points of every
+// thread before pass 1 (all threads call it); on return q[k0] of thread t < T3 holds output point
+// k = t + T3*k0 (T3 = B*C, the block size for every supported shape). ph is the
 // twiddle table of e^{+2 pi i m / (2M)}. Each exchange moves the real parts
 // of all values through LDS, then the imaginary parts, so at most half of a
 // thread's values are in flight (and the LDS holds one component of the
 // whole transform).
-template <int A, int B, int C, int DIR, class Load, class Store>
-__device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, Load load, Store store)
+struct NoPrep {
+    template <class V>
+    __device__ void operator()(V&, bool) const {}
+};
+
+template <int A, int B, int C, int DIR, class Load, class Prep = NoPrep>
+__device__ __forceinline__ void fft3_core(const float2* __restrict__ ph, float* lds, Load load, float2 (&q)[A],
+                                          Prep prep = Prep())
 {
     using S = FftShape<A, B, C>;
     constexpr int M = S::M;
@@ -110,6 +118,9 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
     if (act1) {
 #pragma unroll
         for (int n2 = 0; n2 < C; ++n2) v[n2] = load(t + S::T1 * n2);
+    }
+    prep(v, act1);  // every thread (it may use LDS and barriers)
+    if (act1) {
         dft_reg<C, DIR>(v);
         const float2 w1 = table_tw<DIR>(ph, 2u * (uint32_t)t, MASK);
         float2 tw = make_float2(1.f, 0.f);
@@ -158,7 +169,6 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
     }
 
     // exchange 2 -> 3: element (n0, k1, k2) at k2*(B*A + G2) + k1*A + n0
-    float2 q[A];
     const int k2_3 = t % C, k1_3 = t / C;
     const bool act3 = t < S::T3;
 #pragma unroll
@@ -180,44 +190,113 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
         }
     }
     // pass 3: DFT over n0; output k = k2 + C k1 + C B k0 = t + T3 k0
-    if (act3) {
-        dft_reg<A, DIR>(q);
+    if (act3) dft_reg<A, DIR>(q);
+}
+
+// The engine with per-point output: store(k, value) receives output point k.
+template <int A, int B, int C, int DIR, class Load, class Store>
+__device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, Load load, Store store)
+{
+    float2 q[A];
+    fft3_core<A, B, C, DIR>(ph, lds, load, q);
+    const int t = threadIdx.x;
+    if (t < FftShape<A, B, C>::T3) {
 #pragma unroll
-        for (int k0 = 0; k0 < A; ++k0) store(t + S::T3 * k0, q[k0]);
+        for (int k0 = 0; k0 < A; ++k0) store(t + FftShape<A, B, C>::T3 * k0, q[k0]);
     }
 }
 
-// Forward: channel c, chunk i of the channel-major 8-bit rows; writes Z.
+// Forward: channel c, chunk i of the channel-major 8-bit rows; writes the
+// real spectrum X[0..M] of the chunk (M+1 bins, row pitch Xp). The packed
+// transform Z of z[m] = x[2m] + i x[2m+1] is paired in the epilogue,
+//   X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]),
+// with Z[M-k] taken from the thread that holds it through LDS: point
+// k = t + T3*k0 has its mirror M-k at element A-1-k0 of thread T3-t (t > 0; for
+// t = 0 at element (A-k0) mod A of thread 0). Two rounds per component, each
+// moving half the elements, keep the extra registers to A/2 per component.
 template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_forward_k(
     const uint8_t* __restrict__ rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
     size_t period, const float* __restrict__ chan_scale, const float2* __restrict__ ph,
-    float2* __restrict__ Z, size_t nchunk, size_t Zp)
+    float2* __restrict__ X, size_t nchunk, size_t Xp, uint32_t nbits)
 {
     using S = FftShape<A, B, C>;
+    static_assert(S::T3 == NT, "pass 3 uses every thread");
+    constexpr int M = S::M, T3 = S::T3, H = A / 2;
+    static_assert(S::LDS_FLOATS >= M, "a pairing round fits the LDS");
     __shared__ float lds[S::LDS_FLOATS];
     const size_t chunk = blockIdx.x, c = blockIdx.y;
     const uint8_t* src = rows + c * row_bytes;
     const size_t t0 = chunk * chunk_step;
     const float s = chan_scale[c];
-    float2* dst = Z + (c * nchunk + chunk) * Zp;
+    float2* dst = X + (c * nchunk + chunk) * Xp;
     // x[t] = (u8 - 127.5) * scale (src/fdd/unpack/unpack_kernel.cuh:60) for
     // t < nvalid, zero beyond (the zero padding of src/fdd/FDDGPUPlan.cpp:476-483);
-    // with period > 0 the series is read periodically (t mod period).
+    // with period > 0 the series is read periodically (t mod period). Sub-byte
+    // samples (an extension: the reference FDD reads 8 bits only) come packed
+    // LSB-first in 32-bit words and are centred on (2^nbits - 1)/2 likewise.
+    const float mid = 0.5f * (float)((1u << nbits) - 1u);
+    const uint32_t lspw = 5 - __builtin_ctz(nbits);  // log2 samples per word
+    const uint32_t mask = (1u << nbits) - 1u;
+    const uint32_t* wsrc = reinterpret_cast<const uint32_t*>(src);
+    auto value = [&](size_t t) -> uint32_t {
+        if (nbits == 8) return src[t];
+        return (wsrc[t >> lspw] >> ((uint32_t)(t & ((1u << lspw) - 1u)) * nbits)) & mask;
+    };
     auto sample = [&](size_t t) -> float {
         if (period) t %= period;
-        return (t < nvalid) ? ((float)src[t] - 127.5f) * s : 0.f;
+        return (t < nvalid) ? ((float)value(t) - mid) * s : 0.f;
     };
     auto load = [&](int n) -> float2 {
         const size_t t = t0 + 2 * (size_t)n;
         if (!period && t + 1 < nvalid) {
-            const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t);
-            return make_float2(((float)(pair & 0xff) - 127.5f) * s, ((float)(pair >> 8) - 127.5f) * s);
+            if (nbits == 8) {
+                const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t);
+                return make_float2(((float)(pair & 0xff) - mid) * s, ((float)(pair >> 8) - mid) * s);
+            }
+            // t even: both samples sit in one word
+            const uint32_t w = wsrc[t >> lspw] >> ((uint32_t)(t & ((1u << lspw) - 1u)) * nbits);
+            return make_float2(((float)(w & mask) - mid) * s, ((float)((w >> nbits) & mask) - mid) * s);
         }
         return make_float2(sample(t), sample(t + 1));
     };
-    auto store = [&](int k, float2 val) { dst[k] = val; };
-    fft3<A, B, C, -1>(ph, lds, load, store);
+    float2 q[A];
+    fft3_core<A, B, C, -1>(ph, lds, load, q);
+
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // round h: elements [H(1-h), H(2-h)) go to LDS, elements [hH, hH+H)
+        // read their mirrors (slot (H-1-j)*T3 + T3-t for element hH+j)
+        const int wbase = h ? 0 : H, rbase = h ? H : 0;
+        float mx[H], my[H];
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < H; ++e) lds[e * T3 + t] = comp ? q[wbase + e].y : q[wbase + e].x;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < H; ++j) {
+                const float v = lds[(H - 1 - j) * T3 + (T3 - t)];  // in bounds: < 2*H*T3 = M
+                const float own = comp ? q[rbase].y : q[rbase].x;  // t = 0, j = 0: own mirror
+                if (comp) my[j] = (t == 0 && j == 0) ? own : v;
+                else mx[j] = (t == 0 && j == 0) ? own : v;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int k = t + T3 * (rbase + j);
+            const float2 a = q[rbase + j];
+            const float2 pk = ph[k];  // W^k = conj(pk)
+            const float ux = a.x + mx[j], uy = a.y - my[j];  // Z[k] + conj Z[M-k]
+            const float vx = a.x - mx[j], vy = a.y + my[j];  // Z[k] - conj Z[M-k]
+            // -i/2 W^k v = 1/2 (Im(W^k v), -Re(W^k v))
+            const float wr = vx * pk.x + vy * pk.y, wi = vy * pk.x - vx * pk.y;
+            dst[k] = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
+        }
+    }
+    if (t == 0) dst[M] = make_float2(q[0].x - q[0].y, 0.f);  // X[M] = Re Z[0] - Im Z[0]
 }
 
 // Inverse: DM row j, chunk i of Y (M+1 bins); writes the first `good`
@@ -254,12 +333,15 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
 
 template <int A, int B, int C>
 hipError_t launch_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
-                          size_t period, const float* chan_scale, const float2* ph, float2* Z,
-                          size_t nchan, size_t nchunk, size_t Zp, hipStream_t stream)
+                          size_t period, const float* chan_scale, const float2* ph, float2* X,
+                          size_t nchan, size_t nchunk, size_t Xp, unsigned nbits, hipStream_t stream)
 {
+    if (Xp < (size_t)FftShape<A, B, C>::M + 1) return hipErrorInvalidValue;
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return hipErrorInvalidValue;
+    if (nbits != 8 && (row_bytes % 4)) return hipErrorInvalidValue;
     hipLaunchKernelGGL((fdd_forward_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nchan),
                        dim3(FftShape<A, B, C>::T), 0, stream, rows, row_bytes, chunk_step, nvalid,
-                       period, chan_scale, ph, Z, nchunk, Zp);
+                       period, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits);
     return hipGetLastError();
 }
 
@@ -376,10 +458,10 @@ extern "C" int dd_fdd_fft_supported(size_t nfft)
 
 extern "C" hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step,
                                      size_t nvalid, size_t period, const float* chan_scale,
-                                     const float2* ph, size_t nfft, float2* Z, size_t nchan,
-                                     size_t nchunk, size_t Zp, hipStream_t stream)
+                                     const float2* ph, size_t nfft, float2* X, size_t nchan,
+                                     size_t nchunk, size_t Xp, unsigned nbits, hipStream_t stream)
 {
-#define DD_FWD(A, B, C) launch_forward<A, B, C>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, Z, nchan, nchunk, Zp, stream)
+#define DD_FWD(A, B, C) launch_forward<A, B, C>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, X, nchan, nchunk, Xp, nbits, stream)
     switch (nfft) {
     case 8192:  return DD_FWD(16, 16, 16);
     case 16384: return DD_FWD(16, 16, 32);

@@ -41,3 +41,11 @@ CFG2 = dict(nchans=1024, f0=1530.0, df=-0.3125, dt=54.6e-6, nsamps=1 << 20, nbit
             mean=8.0, sigma=2.5, dms=("generate", 0.0, 1000.0, 40.0, 1.25))
 CFG3 = dict(nchans=1024, f0=1530.0, df=-0.3125, dt=54.6e-6, nsamps=1 << 20, nbits=8,
             mean=128.0, sigma=20.0, dms=("linear", 0.0, 0.5, 2048))
+# Many beams of a smaller telescope, 2-bit samples (nsamps log-uniform in
+# 2^17..2^20 per beam); 1024 linear trials from 0 to 2000 (step 2000/1023).
+CFG4 = dict(nchans=512, f0=1500.0, df=-0.78125, dt=81.92e-6, nbits=2, mean=1.5, sigma=0.8,
+            beams=64, nsamps_range=(1 << 17, 1 << 20), dms=("linspace", 0.0, 2000.0, 1024))
+# Streaming: 32 beams delivering 1 s chunks (15625 samples at 64 us), 8-bit,
+# 3000 linear trials from 0 to 1500.
+CFG5 = dict(nchans=1024, f0=1530.0, df=-0.3125, dt=64e-6, nbits=8, mean=128.0, sigma=20.0,
+            beams=32, chunk=15625, dms=("linspace", 0.0, 1500.0, 3000))

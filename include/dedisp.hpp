@@ -181,8 +181,8 @@ public:
     dedisp_size output_length(dedisp_size nsamps) const;
 
 private:
-    void execute_fdd(size_type nsamps, const byte_type* in, byte_type* out, unsigned flags,
-                     bool fourier);
+    void execute_fdd(size_type nsamps, const byte_type* in, size_type in_nbits, byte_type* out,
+                     unsigned flags, bool fourier);
 
     std::unique_ptr<FDDState> m_state;
 };

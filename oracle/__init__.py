@@ -53,10 +53,10 @@ def lib() -> C.CDLL:
                                  C.c_uint, C.c_ulong]
         l.oracle_tdd.restype = C.c_int
         l.oracle_fdd_timeseries.argtypes = [C.c_ulong, C.c_void_p, C.c_void_p, C.c_ulong, C.c_ulong,
-                                            C.c_ulong, C.c_void_p, C.c_void_p]
+                                            C.c_ulong, C.c_void_p, C.c_double, C.c_void_p]
         l.oracle_fdd_timeseries.restype = C.c_int
         l.oracle_fdd_circular.argtypes = [C.c_ulong, C.c_void_p, C.c_void_p, C.c_ulong, C.c_ulong,
-                                          C.c_ulong, C.c_void_p, C.c_void_p]
+                                          C.c_ulong, C.c_void_p, C.c_double, C.c_void_p]
         l.oracle_fdd_circular.restype = C.c_int
         l.oracle_fdd_cpu_asis.argtypes = [C.c_ulong, C.c_float, C.c_void_p, C.c_void_p, C.c_ulong,
                                           C.c_ulong, C.c_ulong, C.c_ulong, C.c_void_p, C.c_void_p,
@@ -144,26 +144,30 @@ def tdd(nchans, delays, killmask, dms, nsamps, max_delay_, data, in_nbits, out_n
 # FDD
 # ---------------------------------------------------------------------------
 
-def fdd_timeseries(nchans, delays, dms, nsamps, max_delay_, data_u8) -> np.ndarray:
+def fdd_timeseries(nchans, delays, dms, nsamps, max_delay_, data_u8, nbits: int = 8) -> np.ndarray:
     """FP64 FDD time series (src/fdd/FDDGPUPlan.cpp:618-636 time-series mode,
-    fdd_kernel.cuh:226-229 truncated delays): float64 [ndm][nsamps-max_delay]."""
+    fdd_kernel.cuh:226-229 truncated delays): float64 [ndm][nsamps-max_delay].
+    data_u8: one sample value per byte [nsamps][nchans]; samples enter as
+    (u - (2^nbits - 1)/2)/nchans (127.5 for the reference's 8-bit input)."""
     dms = np.ascontiguousarray(dms, np.float32)
     delays = np.ascontiguousarray(delays, np.float32)
     data_u8 = np.ascontiguousarray(data_u8, np.uint8)
     out = np.zeros((dms.size, nsamps - max_delay_), np.float64)
     rc = lib().oracle_fdd_timeseries(nchans, _p(delays), _p(dms), dms.size, nsamps, max_delay_,
-                                     _p(data_u8), _p(out))
+                                     _p(data_u8), ((1 << nbits) - 1) / 2.0, _p(out))
     assert rc == 0
     return out
 
 
-def fdd_circular(nchans, delays, dms, nsamps, N, data_u8) -> np.ndarray:
-    """FP64 circular shift-sum over length N: float64 [ndm][N]."""
+def fdd_circular(nchans, delays, dms, nsamps, N, data_u8, nbits: int = 8) -> np.ndarray:
+    """FP64 circular shift-sum over length N: float64 [ndm][N] (offset as in
+    fdd_timeseries)."""
     dms = np.ascontiguousarray(dms, np.float32)
     delays = np.ascontiguousarray(delays, np.float32)
     data_u8 = np.ascontiguousarray(data_u8, np.uint8)
     out = np.zeros((dms.size, N), np.float64)
-    rc = lib().oracle_fdd_circular(nchans, _p(delays), _p(dms), dms.size, nsamps, N, _p(data_u8), _p(out))
+    rc = lib().oracle_fdd_circular(nchans, _p(delays), _p(dms), dms.size, nsamps, N, _p(data_u8),
+                                   ((1 << nbits) - 1) / 2.0, _p(out))
     assert rc == 0
     return out
 

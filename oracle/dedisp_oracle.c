@@ -222,11 +222,13 @@ int oracle_fdd_delay(float dm, float delay)
  * with d_c = (int)(dm_i * delay[c]) (fdd_kernel.cuh:226-229), t < nsamps-max_delay.
  * Zero-padded transforms of length N >= nsamps make the Fourier shift a plain
  * shift for these t (circular shift-sum identity). Output float64 rows of
- * nsamps-max_delay values. 8-bit input only (src/fdd/unpack/unpack_kernel.cuh:60). */
+ * nsamps-max_delay values. in holds one sample value per byte; offset is 127.5
+ * for the reference's 8-bit input (src/fdd/unpack/unpack_kernel.cuh:60) and
+ * (2^nbits - 1)/2 for the sub-byte extension. */
 int oracle_fdd_timeseries(unsigned long nchans, const float* delay_table,
                           const float* dm_list, unsigned long ndm,
                           unsigned long nsamps, unsigned long max_delay,
-                          const unsigned char* in, double* out)
+                          const unsigned char* in, double offset, double* out)
 {
     if (nsamps < max_delay) return -1;
     unsigned long nsc = nsamps - max_delay;
@@ -241,7 +243,7 @@ int oracle_fdd_timeseries(unsigned long nchans, const float* delay_table,
             for (unsigned long t = 0; t < nsc; ++t) {
                 long ts = (long)t + d;
                 if (ts >= 0 && (unsigned long)ts < nsamps)
-                    orow[t] += ((double)in[(unsigned long)ts * nchans + c] - 127.5) * inv;
+                    orow[t] += ((double)in[(unsigned long)ts * nchans + c] - offset) * inv;
             }
         }
     }
@@ -256,7 +258,7 @@ int oracle_fdd_timeseries(unsigned long nchans, const float* delay_table,
 int oracle_fdd_circular(unsigned long nchans, const float* delay_table,
                         const float* dm_list, unsigned long ndm,
                         unsigned long nsamps, unsigned long N,
-                        const unsigned char* in, double* out)
+                        const unsigned char* in, double offset, double* out)
 {
     if (nsamps > N) return -1;
     double inv = 1.0 / (double)nchans;
@@ -270,7 +272,7 @@ int oracle_fdd_circular(unsigned long nchans, const float* delay_table,
             unsigned long dm = (unsigned long)(((d % (long)N) + (long)N) % (long)N);
             for (unsigned long ts = 0; ts < nsamps; ++ts) {
                 unsigned long t = (ts + N - dm) % N;
-                orow[t] += ((double)in[ts * nchans + c] - 127.5) * inv;
+                orow[t] += ((double)in[ts * nchans + c] - offset) * inv;
             }
         }
     }

@@ -123,7 +123,7 @@ def test_fdd_validation(dd):
     out = np.zeros(100000, np.float32)
     assert plan.execute_raw(1000, buf, 8, out, 32) == dd.DEDISP_NO_DM_LIST_SET
     plan.set_dm_list(np.array([0.0, 5.0], f32))
-    assert plan.execute_raw(1000, buf, 4, out, 32) == dd.DEDISP_UNSUPPORTED_IN_NBITS
+    assert plan.execute_raw(1000, buf, 16, out, 32) == dd.DEDISP_UNSUPPORTED_IN_NBITS
     assert plan.execute_raw(1000, buf, 8, out, 16) == dd.DEDISP_UNSUPPORTED_OUT_NBITS
     assert plan.execute_raw(1000, None, 8, out, 32) == dd.DEDISP_INVALID_POINTER
     # FDD plans have no gulp / adv / guru interface (src/clib/dedisp.cpp:182, 428, 488)
@@ -206,3 +206,25 @@ def test_fdd_fourier_vs_oracle(dd, orc, nsamps, nch):
     ref = orc.fdd_fourier(nch, delays, dms, nsamps, plan.max_delay, data)
     assert out.shape == ref.shape
     assert rel_l2(out, ref) < REL_FP32
+
+
+@pytest.mark.parametrize("nbits", [2, 4])
+def test_fdd_subbyte_input_vs_oracle(dd, orc, nbits):
+    """Sub-byte FDD input (an extension: the reference FDD reads 8 bits only):
+    samples centred on (2^nbits - 1)/2, here on the many-beam geometry
+    (512 ch 1500 MHz down in 0.78125 MHz, dt 81.92 us) with a shorter DM range."""
+    from dedisp_tests_amd import synth
+    nch, dt, f0, df = 512, 81.92e-6, 1500.0, -0.78125
+    nsamps = 1 << 16
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    vmax = (1 << nbits) - 1
+    s = synth.filterbank(nsamps, nch, nbits, vmax / 2.0, vmax / 4.0, seed=nbits,
+                         pulses=[(150.0, 20000, 3, 2)], delays=delays)
+    packed = dd.pack_samples(s, nbits)
+    dms = np.linspace(0, 300, 13, dtype=f32)
+    for prec in (dd.FDD_FP32, dd.FDD_BF16):
+        plan = _fdd_plan(dd, nch, dt, f0, df, prec=prec)
+        plan.set_dm_list(dms)
+        out = plan.execute(packed, nbits, 32)
+        ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, s.astype(np.uint8), nbits=nbits)
+        assert rel_l2(out, ref) < REL_FP32

@@ -25,7 +25,7 @@ def test_tdd_stream_matches_one_shot(dd):
     plan = dd.Plan(nch, dt, f0, df, impl=dd.TDD)
     plan.set_dm_list(np.arange(0, 200, 2, dtype=f32))
     hist = plan.max_delay
-    chunk, npush = 1500, 6  # chunk < max_delay exercises a history longer than a chunk
+    chunk, npush = 500, 6  # chunk < max_delay (713) exercises a history longer than a chunk
     assert chunk < hist
     rng = np.random.default_rng(5)
     series = [rng.integers(0, 16, size=(chunk * npush, nch), dtype=np.uint32) for _ in range(2)]
@@ -108,3 +108,41 @@ def test_stream_admission_and_errors(dd):
     assert st.push_raw(2, buf, out) == dd.DEDISP_INVALID_POINTER  # beam out of range
     plan.set_dm_list(np.array([0.0, 20.0], f32))                  # max_delay changes
     assert st.push_raw(0, buf, out) == dd.DEDISP_INVALID_PLAN
+
+
+def test_stream_cfg5_geometry(dd, orc):
+    """The streaming workload's geometry (1024 ch, 3000 trials 0..1500, 1 s
+    chunks of 15625 samples at 64 us, max_delay 24849 > chunk) on two beams:
+    three pushes each equal one-shot execution bit for bit, and two DM rows
+    equal the oracle."""
+    import torch
+    from dedisp_tests_amd import synth
+    g = synth.CFG5
+    nch, dt, f0, df, chunk = g["nchans"], g["dt"], g["f0"], g["df"], g["chunk"]
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.TDD)
+    dms = np.linspace(0.0, 1500.0, 3000, dtype=f32)
+    plan.set_dm_list(dms)
+    hist = plan.max_delay
+    assert hist == 24849
+    npush = 3
+    st = dd.Stream(plan, nbeams=2, chunk_nsamps=chunk, in_nbits=8, out_nbits=32)
+    d_out = torch.empty((3000, chunk), dtype=torch.float32, device="cuda")
+    for beam in (0, 1):
+        rng = np.random.default_rng(100 + beam)
+        data = np.clip(np.rint(rng.normal(128, 20, size=(chunk * npush, nch))), 0, 255).astype(np.uint8)
+        outs = []
+        for n in range(npush):
+            d_in = torch.from_numpy(np.ascontiguousarray(data[n * chunk:(n + 1) * chunk])).cuda()
+            torch.cuda.synchronize()
+            assert st.push_raw(beam, d_in, d_out, dd.DEVICE_POINTERS) == 0
+            outs.append(d_out.cpu().numpy().copy())
+        got = _stitch(outs, chunk, hist)
+        d_all = torch.from_numpy(data).cuda()
+        one = torch.empty((3000, chunk * npush - hist), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        assert plan.execute_raw(chunk * npush, d_all, 8, one, 32, dd.DEVICE_POINTERS) == 0
+        assert np.array_equal(got, one.cpu().numpy())
+        delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+        for i in (1, 2999):
+            ref = orc.tdd(nch, delays, None, dms, chunk * npush, hist, data, 8, 32, first_dm=i, dm_count=1)
+            assert np.array_equal(got[i], ref[0]), (beam, i)

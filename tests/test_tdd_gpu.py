@@ -245,3 +245,34 @@ def test_tdd_cfg2_full_size_properties(dd, orc):
     for i in (1, 583, 1166):
         ref = orc.tdd(nch, delays, None, dms, nsamps, plan.max_delay, data, 4, 32, first_dm=i, dm_count=1)
         assert np.array_equal(d_out[i].cpu().numpy(), ref[0]), i
+
+
+@pytest.mark.parametrize("nsamps", [1 << 17, 1 << 18])
+def test_tdd_cfg4_beams_2bit(dd, orc, nsamps):
+    """512 ch 1500 MHz down in 0.78125 MHz, dt 81.92 us, 2-bit N(1.5, 0.8),
+    1024 trials 0..2000: the shortest beams of the many-beam workload at full
+    geometry. DM 0 equals the channel sum; three DM rows equal the oracle."""
+    import torch
+    from dedisp_tests_amd import synth
+    g = synth.CFG4
+    nch, dt, f0, df = g["nchans"], g["dt"], g["f0"], g["df"]
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    s = synth.filterbank(nsamps, nch, 2, g["mean"], g["sigma"], seed=nsamps,
+                         pulses=[(700.0, 5000, 3, 2)], delays=delays)
+    data = dd.pack_samples(s, 2)
+    plan = _plan(dd, nch, dt, f0, df)
+    dms = np.linspace(0.0, 2000.0, 1024, dtype=f32)
+    plan.set_dm_list(dms)
+    assert plan.max_delay == 38585
+    nsc = nsamps - plan.max_delay
+    d_in = torch.from_numpy(data).cuda()
+    d_out = torch.empty((1024, nsc), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, 2, d_out, 32, dd.DEVICE_POINTERS) == 0
+    row0 = d_out[0].cpu().numpy()
+    sums = s[:nsc].astype(np.int64).sum(axis=1).astype(f32)
+    factor = (f32(3.0) * f32(1024.0)) / f32(255.0) / f32(16.0)
+    assert np.array_equal(row0, ((sums * f32(1.0)) / (f32(3.0) * f32(nch)) * factor).astype(f32))
+    for i in (358, 511, 1023):  # 358: nearest trial to the injected DM 700
+        ref = orc.tdd(nch, delays, None, dms, nsamps, plan.max_delay, data, 2, 32, first_dm=i, dm_count=1)
+        assert np.array_equal(d_out[i].cpu().numpy(), ref[0]), i
