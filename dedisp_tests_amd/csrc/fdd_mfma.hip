@@ -44,6 +44,12 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 
+#ifndef DD_MFMA_LOPOLY
+#define DD_MFMA_LOPOLY 0
+#endif
+#ifndef DD_MFMA_NEGA
+#define DD_MFMA_NEGA 1
+#endif
 #ifndef DD_MFMA_STAGGER
 #define DD_MFMA_STAGGER 1
 #endif
@@ -67,13 +73,16 @@ template <>
 struct Form<0> {  // float32: re, im, -im
     static constexpr int PLANES = 3, ELEM = 4;
 };
+// The bfloat16 and float16 forms stage re and im only when DD_MFMA_NEGA is
+// set (the products with -im then take the negated A operand, two xor per
+// register, instead of a staged -im plane).
 template <>
-struct Form<1> {  // bfloat16: re hi, re lo, im hi, im lo, -im hi, -im lo
-    static constexpr int PLANES = 6, ELEM = 2;
+struct Form<1> {  // bfloat16: re hi, re lo, im hi, im lo [, -im hi, -im lo]
+    static constexpr int PLANES = DD_MFMA_NEGA ? 4 : 6, ELEM = 2;
 };
 template <>
-struct Form<2> {  // float16: re, im, -im
-    static constexpr int PLANES = 3, ELEM = 2;
+struct Form<2> {  // float16: re, im [, -im]
+    static constexpr int PLANES = DD_MFMA_NEGA ? 2 : 3, ELEM = 2;
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
@@ -83,28 +92,18 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
 
-// A 32-bit constant in a scalar register, out of the compiler's reach. The
-// bfloat16 pair (-1, 0) folded into v_dot2c_f32_bf16 as the inline constant
-// -1.0 is not read as (-1, 0) by gfx950 (measured: every remainder wrong),
-// while the same value from a scalar register is.
-__device__ __forceinline__ uint32_t sconst(uint32_t v)
-{
-    uint32_t r;
-    asm("s_mov_b32 %0, %1" : "=s"(r) : "i"(v));
-    return r;
-}
-
 // hi: truncated bfloat16 of a (low half) and b (high half); lo: the rounded
-// bfloat16 of the remainders a - hi(a), b - hi(b), which are exact in float32
-// (formed with a dot product against (-1, 0) / (0, -1)).
+// bfloat16 of the remainders a - hi(a), b - hi(b), which are exact in float32.
+// The remainders are formed with v_and_b32 + v_sub_f32, which issue alongside
+// MFMAs; a dot product against (-1, 0) would take one instruction per value
+// but is serialised with the matrix pipe on gfx950 (measured: every
+// v_dot2c_f32_bf16 adds ~7.5 cycles to a stream of MFMAs, while and/sub
+// under an MFMA are nearly free).
 __device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uint32_t& lo)
 {
     hi = __builtin_amdgcn_perm(fbits(b), fbits(a), 0x07060302u);
-    const bf16x2 h = __builtin_bit_cast(bf16x2, hi);
-    const bf16x2 m_lo = __builtin_bit_cast(bf16x2, sconst(0x0000BF80u));
-    const bf16x2 m_hi = __builtin_bit_cast(bf16x2, sconst(0xBF800000u));
-    const float ra = __builtin_amdgcn_fdot2_f32_bf16(h, m_lo, a, false);
-    const float rb = __builtin_amdgcn_fdot2_f32_bf16(h, m_hi, b, false);
+    const float ra = a - __builtin_bit_cast(float, fbits(a) & 0xffff0000u);
+    const float rb = b - __builtin_bit_cast(float, fbits(b) & 0xffff0000u);
     const bf16x2 l = {(__bf16)ra, (__bf16)rb};
     lo = __builtin_bit_cast(uint32_t, l);
 }
@@ -208,15 +207,17 @@ void fdd_mfma_k(
             p[SLAB / 4] = make_uint2(rl0, rl1);
             p[2 * SLAB / 4] = make_uint2(ih0, ih1);
             p[3 * SLAB / 4] = make_uint2(il0, il1);
-            p[4 * SLAB / 4] = make_uint2(ih0 ^ 0x80008000u, ih1 ^ 0x80008000u);
-            p[5 * SLAB / 4] = make_uint2(il0 ^ 0x80008000u, il1 ^ 0x80008000u);
+            if constexpr (!DD_MFMA_NEGA) {
+                p[4 * SLAB / 4] = make_uint2(ih0 ^ 0x80008000u, ih1 ^ 0x80008000u);
+                p[5 * SLAB / 4] = make_uint2(il0 ^ 0x80008000u, il1 ^ 0x80008000u);
+            }
         } else {
             const uint32_t r0 = pack_f16x2(xr[0], xr[1]), r1 = pack_f16x2(xr[2], xr[3]);
             const uint32_t i0 = pack_f16x2(xi[0], xi[1]), i1 = pack_f16x2(xi[2], xi[3]);
             uint2* p = reinterpret_cast<uint2*>(buf) + off / 4;
             p[0] = make_uint2(r0, r1);
             p[SLAB / 4] = make_uint2(i0, i1);
-            p[2 * SLAB / 4] = make_uint2(i0 ^ 0x80008000u, i1 ^ 0x80008000u);
+            if constexpr (!DD_MFMA_NEGA) p[2 * SLAB / 4] = make_uint2(i0 ^ 0x80008000u, i1 ^ 0x80008000u);
         }
       }
     };
@@ -238,6 +239,21 @@ void fdd_mfma_k(
     // The second half of the waves computes them for the next K-step at the
     // end of the current one (the registers are free by then), so after each
     // barrier one wave of a SIMD issues MFMAs while the other waits on LDS.
+    // With DD_MFMA_LOPOLY the bfloat16 and float16 forms evaluate the low
+    // factor e^{2 pi i j/65536} (j < 256, angle below 0.025) as
+    // (1 - t^2/2 + t^4/24, t - t^3/6), truncation error below 1e-10, instead
+    // of reading T_lo. That halves the random-address LDS reads but measured
+    // slower (bf16 61.4 -> 65.0 ms, fp16 37.4 -> 40.1 ms on 1024 channels x
+    // 2048 DMs x 23 chunks of 65536), so it is off by default.
+    constexpr bool lopoly = DD_MFMA_LOPOLY && PREC != 0;
+    auto t_lo = [](uint32_t j) -> float2 {
+        constexpr float c = 6.283185307179586f / 65536.f;
+        // (2^23 + j) c - 2^23 c in one rounding: j c for integer j < 2^23
+        const float t = __builtin_fmaf(__builtin_bit_cast(float, 0x4B000000u | j), c, -8388608.f * c);
+        const float t2 = t * t;
+        return make_float2(__builtin_fmaf(t2, __builtin_fmaf(t2, 1.f / 24.f, -0.5f), 1.f),
+                           t * __builtin_fmaf(t2, -1.f / 6.f, 1.f));
+    };
     float2 p[8], w[8];
     auto lookups = [&](const u32x4& qq) {
         float2 th[8], tl[8], wh[8], wl[8];
@@ -246,9 +262,14 @@ void fdd_mfma_k(
             const uint32_t q = (qq[s >> 1] >> (16 * (s & 1))) & 0xffffu;
             const uint32_t m8 = __umul24(8u * k0, q), w8 = q << 3;
             th[s] = *reinterpret_cast<const float2*>(tab + ((m8 >> 8) & 0x7f8u));
-            tl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
             wh[s] = *reinterpret_cast<const float2*>(tab + ((w8 >> 8) & 0x7f8u));
-            wl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (w8 & 0x7f8u));
+            if constexpr (lopoly) {
+                tl[s] = t_lo((m8 >> 3) & 255u);
+                wl[s] = t_lo(q & 255u);
+            } else {
+                tl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
+                wl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (w8 & 0x7f8u));
+            }
         }
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
@@ -324,21 +345,37 @@ void fdd_mfma_k(
                 x = X[SLAB / 8];  // re lo
                 ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ar, 0, 0, 0);
                 ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ai, 0, 0, 0);
-                x = X[2 * SLAB / 8];  // im hi
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, x, ai, 0, 0, 0);
-                x = X[4 * SLAB / 8];  // -im hi
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ail, x, ar, 0, 0, 0);
-                x = X[3 * SLAB / 8];  // im lo
-                ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
-                x = X[5 * SLAB / 8];  // -im lo
-                ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
+                if constexpr (DD_MFMA_NEGA) {
+                    // (-Aim) * Xim: negated A operands (sign bits of both halves)
+                    uint32_t nih[4], nil[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) nih[s] = pih[s] ^ 0x80008000u, nil[s] = pil[s] ^ 0x80008000u;
+                    const bf16x8 nAih = __builtin_bit_cast(bf16x8, nih), nAil = __builtin_bit_cast(bf16x8, nil);
+                    x = X[2 * SLAB / 8];  // im hi
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAih, x, ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, x, ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAil, x, ar, 0, 0, 0);
+                    x = X[3 * SLAB / 8];  // im lo
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAih, x, ar, 0, 0, 0);
+                } else {
+                    x = X[2 * SLAB / 8];  // im hi
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, x, ai, 0, 0, 0);
+                    x = X[4 * SLAB / 8];  // -im hi
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ail, x, ar, 0, 0, 0);
+                    x = X[3 * SLAB / 8];  // im lo
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ai, 0, 0, 0);
+                    x = X[5 * SLAB / 8];  // -im lo
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, x, ar, 0, 0, 0);
+                }
                 acc_r[b] = ar;
                 acc_i[b] = ai;
             } else {
                 const f16x8* X = reinterpret_cast<const f16x8*>(bb);
-                const f16x8 xr = X[0], xi = X[SLAB / 8], xn = X[2 * SLAB / 8];
+                const f16x8 xr = X[0], xi = X[SLAB / 8];
                 uint32_t pr[4], pi[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
@@ -348,7 +385,14 @@ void fdd_mfma_k(
                 const f16x8 Ar = __builtin_bit_cast(f16x8, pr), Ai = __builtin_bit_cast(f16x8, pi);
                 acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ar, xr, acc_r[b], 0, 0, 0);
                 acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ar, xi, acc_i[b], 0, 0, 0);
-                acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, xn, acc_r[b], 0, 0, 0);
+                if constexpr (DD_MFMA_NEGA) {
+                    uint32_t ni[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) ni[s] = pi[s] ^ 0x80008000u;
+                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, ni), xi, acc_r[b], 0, 0, 0);
+                } else {
+                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, X[2 * SLAB / 8], acc_r[b], 0, 0, 0);
+                }
                 acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, xr, acc_i[b], 0, 0, 0);
             }
             if (b + 1 < NB) {
@@ -386,6 +430,7 @@ void fdd_mfma_k(
         }
     }
 }
+
 
 }  // namespace
 
