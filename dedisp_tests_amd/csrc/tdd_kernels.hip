@@ -21,6 +21,10 @@
 
 #include "kernels.h"
 
+#ifndef TDD_EARLY_READS
+#define TDD_EARLY_READS 1
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -145,8 +149,12 @@ __global__ __launch_bounds__(256) void tdd_generic_k(
 // Tile: a workgroup owns 64*G*SPW consecutive output samples of WAVES*DPW DMs.
 // Lane l of a wave owns the G input words l, l+64, .., l+64(G-1) of the tile
 // (SPW samples each), so a wave's LDS reads are unit-stride.
-// code[dm][j] = (word offset of the delayed window inside the staged window)
-//               << 5 | bit shift (sub-byte) or byte offset (8-bit).
+// code[group][j][i] (group = DPW consecutive DMs of one wave, DM i of the
+// group, channel j) = (LDS byte offset of the delayed window: staged channel
+// slot j % chunk times the window plus the word offset inside it, times 4)
+// << 5 | bit shift (sub-byte; v_alignbit reads only these 5 bits) or byte
+// offset (8-bit). The 2*DPW codes of a channel pair are consecutive, so a
+// wave fetches them with one scalar load from a pointer that only advances.
 //
 // Lane layout of the packed accumulators for one input word k:
 //   P = 8/NBITS byte-lane registers m (P = 1 for 8-bit); byte b of register m
@@ -184,9 +192,6 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t tw0 = tile * TILE_WORDS;  // first input word of the tile
     const uint32_t dmb = dblk * (WAVES * DPW) + wave * DPW;
-    uint32_t dmi[DPW];
-#pragma unroll
-    for (int i = 0; i < DPW; ++i) dmi[i] = min(dmb + i, ndm - 1);
 
     uint32_t acc8[DPW][G][P];
     uint32_t acc16[DPW][G][P][2];
@@ -251,8 +256,8 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         const uint32_t* p = wbase + lane + 64 * k;
         uint32_t w0 = p[0], w1 = p[1];
         if constexpr (NBITS == 8) {
-            // byte offset -> selectors for bytes (r, r+2) and (r+1, r+3)
-            uint32_t sel = 0x0C000C00u + shift * 0x00010001u + 0x00020000u + (uint32_t)m * 0x00010001u;
+            // byte offset (low 5 bits of the code) -> selectors for bytes (r, r+2) and (r+1, r+3)
+            uint32_t sel = 0x0C000C00u + (shift & 31u) * 0x00010001u + 0x00020000u + (uint32_t)m * 0x00010001u;
             return __builtin_amdgcn_perm(w1, w0, sel);
         } else {
             uint32_t x = __builtin_amdgcn_alignbit(w1, w0, shift);
@@ -269,6 +274,7 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
             } else {
                 const uint32_t* p0 = b0 + lane + 64 * k;
                 const uint32_t* p1 = b1 + lane + 64 * k;
+                // the funnel shift takes its amount modulo 32: the code's low 5 bits
                 uint32_t x0 = __builtin_amdgcn_alignbit(p0[1], p0[0], s0);
                 uint32_t x1 = __builtin_amdgcn_alignbit(p1[1], p1[0], s1);
 #pragma unroll
@@ -290,11 +296,10 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         }
     };
 
-    // Per-DM rows of the code table: wave-uniform 32-bit offsets (scalar loads).
-    uint32_t crow[DPW];
-#pragma unroll
-    for (int i = 0; i < DPW; ++i) crow[i] = __builtin_amdgcn_readfirstlane(dmi[i] * nact);
+    // This wave's rows of the code table (wave-uniform: scalar loads).
+    const uint32_t coff = __builtin_amdgcn_readfirstlane((dblk * WAVES + wave) * nact * DPW);
     const uint32_t wrow = __builtin_amdgcn_readfirstlane(dblk * nact);
+    auto window = [&](uint32_t c) { return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lds) + (c >> 5)); };
 
     // channels per byte-lane group (sub-byte): the byte lanes are widened
     // after every group; 8-bit input goes straight to 16-bit lanes
@@ -314,23 +319,60 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         for (uint32_t g0 = 0; g0 < cc; g0 += GROUP) {
             const uint32_t g1 = min(cc, g0 + GROUP);
             uint32_t jj = g0;
-            for (; jj + 1 < g1; jj += 2) {
-                const uint32_t j = j0 + jj;
-                const uint32_t* l0 = lds + jj * W;
-                const uint32_t* l1 = l0 + W;
+            // codes of the first pair; each iteration fetches the next pair's
+            // (clamped to the last pair of the group: in bounds, unused)
+            uint32_t cn[2 * DPW];
+            if (jj + 1 < g1) {
+                const uint32_t* cp = code + (coff + (j0 + jj) * DPW);
 #pragma unroll
-                for (int i = 0; i < DPW; ++i) {
-                    const uint32_t c0 = code[crow[i] + j], c1 = code[crow[i] + j + 1];
-                    accumulate2(i, l0 + (c0 >> 5), c0 & 31u, l1 + (c1 >> 5), c1 & 31u);
+                for (int t = 0; t < 2 * DPW; ++t) cn[t] = cp[t];
+            }
+            for (; jj + 1 < g1; jj += 2) {
+                uint32_t c[2 * DPW];
+#pragma unroll
+                for (int t = 0; t < 2 * DPW; ++t) c[t] = cn[t];
+                {
+                    // issued before this pair's LDS reads: LDS and scalar loads
+                    // share one counter, so the first wait covers both
+                    const uint32_t jn = (jj + 3 < g1) ? jj + 2 : jj;
+                    const uint32_t* cp = code + (coff + (j0 + jn) * DPW);
+#pragma unroll
+                    for (int t = 0; t < 2 * DPW; ++t) cn[t] = cp[t];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if constexpr (NBITS < 8 && TDD_EARLY_READS) {
+                    // all LDS words of the pair first, so their latencies overlap
+                    uint32_t wv[2 * DPW][G][2];
+#pragma unroll
+                    for (int t = 0; t < 2 * DPW; ++t)
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            const uint32_t* q = window(c[t]) + lane + 64 * k;
+                            wv[t][k][0] = q[0];
+                            wv[t][k][1] = q[1];
+                        }
+#pragma unroll
+                    for (int i = 0; i < DPW; ++i)
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            const uint32_t x0 = __builtin_amdgcn_alignbit(wv[i][k][1], wv[i][k][0], c[i]);
+                            const uint32_t x1 = __builtin_amdgcn_alignbit(wv[DPW + i][k][1], wv[DPW + i][k][0], c[DPW + i]);
+#pragma unroll
+                            for (int m = 0; m < P; ++m)
+                                acc8[i][k][m] += ((x0 >> (m * NBITS)) & M8) + ((x1 >> (m * NBITS)) & M8);
+                        }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < DPW; ++i)
+                        accumulate2(i, window(c[i]), c[i], window(c[DPW + i]), c[DPW + i]);
                 }
             }
             if (jj < g1) {
-                const uint32_t j = j0 + jj;
-                const uint32_t* l0 = lds + jj * W;
+                const uint32_t* cp = code + (coff + (j0 + jj) * DPW);
 #pragma unroll
                 for (int i = 0; i < DPW; ++i) {
-                    const uint32_t c0 = code[crow[i] + j];
-                    accumulate1(i, l0 + (c0 >> 5), c0 & 31u);
+                    const uint32_t c0 = cp[i];
+                    accumulate1(i, window(c0), c0);
                 }
             }
             if constexpr (NBITS < 8) flush8();
@@ -439,8 +481,8 @@ extern "C" hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row
 
 // Instantiated tilings: (nbits, words per lane G, DMs per wave, waves).
 #define DD_LDS_TILINGS(X) \
-    X(1, 1, 2, 4) X(1, 1, 4, 4) \
-    X(2, 1, 2, 4) X(2, 2, 2, 4) X(2, 1, 4, 4) \
+    X(1, 1, 2, 4) X(1, 1, 4, 4) X(1, 1, 2, 8) \
+    X(2, 1, 2, 4) X(2, 2, 2, 4) X(2, 1, 4, 4) X(2, 1, 2, 8) \
     X(4, 2, 2, 4) X(4, 2, 4, 4) X(4, 1, 4, 4) X(4, 2, 2, 8) X(4, 4, 2, 4) X(4, 2, 4, 8) \
     X(4, 2, 1, 4) X(4, 1, 2, 8) \
     X(8, 2, 2, 4) X(8, 4, 2, 4) X(8, 2, 4, 4) X(8, 2, 2, 8)

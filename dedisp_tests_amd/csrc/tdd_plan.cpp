@@ -21,7 +21,7 @@ struct TDDState {
     DeviceBuffer d_in;      // staged input (host-pointer mode)
     DeviceBuffer d_rows;    // channel-major packed rows
     DeviceBuffer d_out;     // staged output (host-pointer mode)
-    DeviceBuffer d_code;    // [ndm][nact] fast path: window offset | shift; generic: delay
+    DeviceBuffer d_code;    // fast path [DM group][nact][DMs per wave] LDS offset | shift; generic [ndm][nact] delay
     DeviceBuffer d_chan;    // [nact] channel index
     DeviceBuffer d_weight;  // [nact] killmask weight (generic path)
     DeviceBuffer d_wstart;  // [ndmblk][nact] staged window start (fast path)
@@ -48,17 +48,20 @@ namespace {
 dd_tdd_tiling default_tiling(unsigned nbits)
 {
     dd_tdd_tiling t{};
-    // Measured on MI355X for 1024 channels x 2^20 4-bit samples x 1167 DMs:
-    // 4 DMs per wave and 16-channel staging chunks took the channel sum from
-    // 67.9 ms (2 DMs, 64 channels) to 29.7 ms; the other widths use the same
-    // shape with the samples-per-lane their tiles are instantiated for.
+    // Measured on MI355X (channel-sum kernel, 2^20 samples, 16-channel
+    // staging chunks): 2 DMs per wave beat 4 at every width once a channel
+    // pair's codes come in one scalar load. 4-bit, 1024 ch x 1167 DMs:
+    // 29.5 ms with 8 waves (4 DMs/wave, 4 waves: 32.4 ms); 8-bit, 1024 ch x
+    // 1000 DMs, 16 samples/lane: 44.4 ms (8 samples/lane, 4 DMs: 50.8 ms);
+    // 2-bit, 512 ch x 1024 DMs: 12.8 ms (13.3 ms); 1-bit, 512 ch x 1024 DMs:
+    // 11.9 ms (13.7 ms).
+    t.dms_per_wave = 2;
     t.waves = 4;
-    t.dms_per_wave = 4;
     switch (nbits) {
     case 1: t.samples_per_lane = 32; break;
     case 2: t.samples_per_lane = 16; break;
-    case 4: t.samples_per_lane = 16; break;
-    default: t.samples_per_lane = 8; break;
+    case 4: t.samples_per_lane = 16; t.waves = 8; break;
+    default: t.samples_per_lane = 16; break;
     }
     t.chunk_channels = 16;
     return t;
@@ -204,15 +207,23 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
             } else {
                 tl.chunk_channels = (unsigned)cc;
                 tl.window_words = (unsigned)W;
-                code.resize(d.size());
-                for (size_t i = 0; i < dm_count; ++i) {
-                    const size_t b = i / dpb;
-                    for (size_t j = 0; j < nact; ++j) {
-                        uint32_t v = d[i * nact + j];
-                        uint32_t rel = v / (uint32_t)spw - wstart[b * nact + j];
-                        uint32_t sh = (nbits == 8) ? (v % 4u) : (v % (uint32_t)spw) * nbits;
-                        code[i * nact + j] = (rel << 5) | sh;
-                    }
+                // code[group][j][i]: DM i of the wave's group of dms_per_wave
+                // (DMs past the end repeat the last one, whose windows lie in
+                // the same last block), LDS byte offset of its delayed window
+                // in staged slot j % cc, shifted left 5, | bit or byte shift
+                const size_t dpw = tl.dms_per_wave, ngroups = ndmblk * tl.waves;
+                code.assign(ngroups * nact * dpw, 0);
+                for (size_t g = 0; g < ngroups; ++g) {
+                    const size_t b = g / tl.waves;
+                    for (size_t j = 0; j < nact; ++j)
+                        for (size_t i = 0; i < dpw; ++i) {
+                            const size_t dm = std::min(g * dpw + i, dm_count - 1);
+                            const uint32_t v = d[dm * nact + j];
+                            const uint32_t rel = v / (uint32_t)spw - wstart[b * nact + j];
+                            const uint32_t sh = (nbits == 8) ? (v % 4u) : (v % (uint32_t)spw) * nbits;
+                            const uint32_t byte = (uint32_t)(((j % cc) * W + rel) * 4);
+                            code[(g * nact + j) * dpw + i] = (byte << 5) | sh;
+                        }
                 }
             }
         }

@@ -92,6 +92,17 @@ def test_tdd_8bit_tilings(dd, orc, monkeypatch, tiling):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("nbits,tiling", [(2, "16,4,4,32"), (2, "16,2,8,32"), (2, "32,2,4,16"),
+                                          (1, "32,4,4,32"), (1, "32,2,8,32")])
+def test_tdd_subbyte_tilings(dd, orc, monkeypatch, nbits, tiling):
+    # 102 DMs (the last block is partly filled: padded groups repeat the last DM) and 200
+    # channels of 2-bit / 1-bit samples (byte lanes flushed every 84 / 254)
+    monkeypatch.setenv("DEDISP_TDD_TILING", tiling)
+    dms = np.arange(0, 61, 0.6, dtype=f32)
+    out, ref = _random_case(dd, orc, nbits, 200, 6000, dms, seed=23 + nbits, geom=(64e-6, 1500.0, -0.78125))
+    assert np.array_equal(out, ref)
+
+
 def test_tdd_cfg1_full_bit_exact(dd, orc):
     # 256 channels 1581.8 MHz down by 0.390625 MHz, dt 64 us, 65536 8-bit samples,
     # one DM-100 pulse, 256 DMs 0..255: bit-exact float output.
