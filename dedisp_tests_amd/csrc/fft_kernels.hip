@@ -105,33 +105,36 @@ struct NoPrep {
 
 template <int A, int B, int C, int DIR, class Load, class Prep = NoPrep>
 __device__ __forceinline__ void fft3_core(const float2* __restrict__ ph, float* lds, Load load, float2 (&q)[A],
-                                          Prep prep = Prep())
+                                          Prep prep = Prep(), int t1 = -1)
 {
     using S = FftShape<A, B, C>;
     constexpr int M = S::M;
     constexpr uint32_t MASK = 2 * M - 1;
     const int t = threadIdx.x;
+    // pass-1 index of this thread (a permutation of 0..T1-1 over the threads
+    // t < T1 may be given; passes 2 and 3 use t itself)
+    const int tp = (t1 < 0) ? t : t1;
 
-    // pass 1: DFT over n2, then twiddle W_M^{t k2} (recurrence reseeded every 8)
+    // pass 1: DFT over n2, then twiddle W_M^{tp k2} (recurrence reseeded every 8)
     float2 v[C];
     const bool act1 = t < S::T1;
     if (act1) {
 #pragma unroll
-        for (int n2 = 0; n2 < C; ++n2) v[n2] = load(t + S::T1 * n2);
+        for (int n2 = 0; n2 < C; ++n2) v[n2] = load(tp + S::T1 * n2);
     }
     prep(v, act1);  // every thread (it may use LDS and barriers)
     if (act1) {
         dft_reg<C, DIR>(v);
-        const float2 w1 = table_tw<DIR>(ph, 2u * (uint32_t)t, MASK);
+        const float2 w1 = table_tw<DIR>(ph, 2u * (uint32_t)tp, MASK);
         float2 tw = make_float2(1.f, 0.f);
 #pragma unroll
         for (int k2 = 1; k2 < C; ++k2) {
-            if (k2 % 8 == 0) tw = table_tw<DIR>(ph, 2u * (uint32_t)t * (uint32_t)k2, MASK);
+            if (k2 % 8 == 0) tw = table_tw<DIR>(ph, 2u * (uint32_t)tp * (uint32_t)k2, MASK);
             else tw = cmul(tw, w1);
             v[k2] = cmul(v[k2], tw);
         }
     }
-    const int n0_1 = t % A, n1_1 = t / A;
+    const int n0_1 = tp % A, n1_1 = tp / A;
 
     // exchange 1 -> 2: element (n0, n1, k2) at (k2*A + n0)*(B+1) + n1
     float2 u[B];
@@ -300,35 +303,90 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
 }
 
 // Inverse: DM row j, chunk i of Y (M+1 bins); writes the first `good`
-// samples of the chunk's real series, times 1/nfft, into out.
+// samples of the chunk's real series, times 1/nfft, into out. Pass 1 takes
+// the packed spectrum of the even/odd samples,
+//   Zu[k] = (Y[k] + conj Y[M-k]) + i e^{+2 pi i k/nfft} (Y[k] - conj Y[M-k]).
+// Every bin is read from HBM once: pass-1 index tp (points k = tp + T1 n2)
+// runs on lane i of a wave and T1 - tp on lane i + 32 of the same wave, so
+// the mirror Y[M-k] of element n2, element C-1-n2 of index T1 - tp, comes
+// from the other half of the wave (v_permlane32_swap). Index T1/2 is its own
+// mirror partner (element C-1-n2 of itself); index 0 pairs element n2 with
+// its own element C-n2 (k = 0 with Y[M]), read back from a small LDS area.
 template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_inverse_k(
     const float2* __restrict__ Y, size_t nchunk, size_t Yp, const float2* __restrict__ ph,
     float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out)
 {
     using S = FftShape<A, B, C>;
-    constexpr int M = S::M;
+    constexpr int M = S::M, T1 = S::T1;
+    static_assert(T1 % 64 == 0 && C % 2 == 0, "pass-1 threads fill whole waves");
+    static_assert(S::LDS_FLOATS >= 2 * C, "index 0 stages its points");
     __shared__ float lds[S::LDS_FLOATS];
     const size_t chunk = blockIdx.x, row = blockIdx.y;
     const float2* src = Y + (row * nchunk + chunk) * Yp;
     float* dst = out + row * out_stride;
     const size_t t0 = chunk * good;
     const float inv_n = 1.0f / (float)(2 * M);
-    auto load = [&](int k) -> float2 {
-        // Zu[k] = (Y[k] + conj Y[M-k]) + i e^{+2 pi i k/nfft} (Y[k] - conj Y[M-k])
-        float2 a = src[k];
-        float2 b = conjf2(src[M - k]);
-        if (k == 0) a.y = 0.f, b = make_float2(src[M].x, 0.f);
-        float2 e = cadd(a, b), d = csub(a, b);
-        float2 o = cmul(d, ph[k]);
+    const float ymx = src[M].x;  // Y[M]: its imaginary part is not used
+    const int t = threadIdx.x;
+    const int hi = (t >> 5) & 1;
+    const int base = (t >> 6) * 32 + (t & 31);
+    const int tp = (t < T1) ? (hi ? (base ? T1 - base : T1 / 2) : base) : t;
+    const bool self0 = (tp == 0), selfh = (tp == T1 / 2);
+    // the value of the lane i ^ 32
+    auto other_half = [&](float x) -> float {
+        const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
+                                                        __builtin_bit_cast(uint32_t, x), false, false);
+        return __builtin_bit_cast(float, hi ? r[0] : r[1]);
+    };
+    // e^{+2 pi i k/nfft} for k = tp + T1 n2 is e^{+2 pi i tp/nfft} e^{i pi n2/C}:
+    // one table read per thread, the second factor a constant per element
+    const float2 ptp = ph[tp];
+    auto zu = [&](int n2, float2 a, float2 ym) -> float2 {  // ym = Y[M-k]
+        const int k = tp + T1 * n2;
+        float2 b = conjf2(ym);
+        if (k == 0) a.y = 0.f, b = make_float2(ymx, 0.f);
+        const float2 e = cadd(a, b), d = csub(a, b);
+        const double ang = M_PI * (double)n2 / (double)C;
+        const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+        const float2 o = cmul(d, n2 ? cmul(ptp, w) : ptp);
         return make_float2(e.x - o.y, e.y + o.x);
     };
-    auto store = [&](int m, float2 val) {
-        const size_t ta = 2 * (size_t)m, tb = ta + 1;
-        if (ta < good && t0 + ta < nsamps_out) dst[t0 + ta] = val.x * inv_n;
-        if (tb < good && t0 + tb < nsamps_out) dst[t0 + tb] = val.y * inv_n;
+    auto load = [&](int k) -> float2 { return src[k]; };
+    auto pair = [&](float2 (&v)[C], bool act) {
+        if (!act) return;
+        if (self0) {
+#pragma unroll
+            for (int e = 0; e < C; ++e) lds[e] = v[e].x, lds[C + e] = v[e].y;
+        }
+#pragma unroll
+        for (int n2 = 0; n2 < C / 2; ++n2) {
+            const int m2 = C - 1 - n2;
+            const float2 a1 = v[n2], a2 = v[m2];
+            // mirrors: element m2 of the partner for n2, element n2 for m2
+            float2 b1 = make_float2(other_half(a2.x), other_half(a2.y));
+            float2 b2 = make_float2(other_half(a1.x), other_half(a1.y));
+            if (selfh) b1 = a2, b2 = a1;
+            if (self0) {
+                if (n2) b1 = make_float2(lds[C - n2], lds[2 * C - n2]);
+                b2 = make_float2(lds[n2 + 1], lds[C + n2 + 1]);
+            }
+            v[n2] = zu(n2, a1, b1);
+            v[m2] = zu(m2, a2, b2);
+            __builtin_amdgcn_sched_barrier(0);  // one pair's temporaries at a time
+        }
     };
-    fft3<A, B, C, +1>(ph, lds, load, store);
+    float2 q[A];
+    fft3_core<A, B, C, +1>(ph, lds, load, q, pair, tp);
+    if (t < S::T3) {
+#pragma unroll
+        for (int k0 = 0; k0 < A; ++k0) {
+            const size_t m = (size_t)t + (size_t)S::T3 * k0;
+            const size_t ta = 2 * m, tb = ta + 1;
+            if (ta < good && t0 + ta < nsamps_out) dst[t0 + ta] = q[k0].x * inv_n;
+            if (tb < good && t0 + tb < nsamps_out) dst[t0 + tb] = q[k0].y * inv_n;
+        }
+    }
 }
 
 template <int A, int B, int C>
