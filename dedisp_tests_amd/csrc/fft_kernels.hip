@@ -367,13 +367,17 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
             float2 b1 = make_float2(other_half(a2.x), other_half(a2.y));
             float2 b2 = make_float2(other_half(a1.x), other_half(a1.y));
             if (selfh) b1 = a2, b2 = a1;
-            if (self0) {
-                if (n2) b1 = make_float2(lds[C - n2], lds[2 * C - n2]);
-                b2 = make_float2(lds[n2 + 1], lds[C + n2 + 1]);
-            }
             v[n2] = zu(n2, a1, b1);
             v[m2] = zu(m2, a2, b2);
             __builtin_amdgcn_sched_barrier(0);  // one pair's temporaries at a time
+        }
+        if (self0) {  // index 0 (one lane): its own element C-n2, from the staged copy
+#pragma unroll
+            for (int n2 = 0; n2 < C; ++n2) {
+                const float2 a = make_float2(lds[n2], lds[C + n2]);
+                const float2 b = n2 ? make_float2(lds[C - n2], lds[2 * C - n2]) : a;
+                v[n2] = zu(n2, a, b);
+            }
         }
     };
     float2 q[A];
