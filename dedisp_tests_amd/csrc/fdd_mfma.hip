@@ -32,6 +32,7 @@
 // over channels whatever the hardware's k order. Accumulator element r of lane
 // l is row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "fdd_kernels.h"
 
@@ -49,6 +50,9 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 #endif
 #ifndef DD_MFMA_NEGA
 #define DD_MFMA_NEGA 1
+#endif
+#ifndef DD_MFMA_BPREF
+#define DD_MFMA_BPREF 1
 #endif
 #ifndef DD_MFMA_STAGGER
 #define DD_MFMA_STAGGER 1
@@ -299,9 +303,27 @@ void fdd_mfma_k(
 
         if (!late) lookups(qv);
 
+        // spectra planes of a bin (bfloat16: re hi, re lo, im hi, im lo;
+        // float16: re, im); with DD_MFMA_BPREF the next bin's are read before
+        // the current bin's MFMAs, so their LDS latency is not waited on
+        constexpr bool bpref = DD_MFMA_BPREF && DD_MFMA_NEGA && PREC != 0;
+        using Pl = typename std::conditional<PREC == 1, bf16x8, f16x8>::type;
+        constexpr int NPL = (PREC == 1) ? 4 : 2;
+        auto planes = [&](int b, Pl (&x)[NPL]) {
+            const Pl* X = reinterpret_cast<const Pl*>(cur + (size_t)(b * 2 * NJ * 8 + b_off) * F::ELEM);
+#pragma unroll
+            for (int t = 0; t < NPL; ++t) x[t] = X[t * SLAB / 8];
+        };
+        Pl xc[NPL];
+        if constexpr (bpref) planes(0, xc);
+
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const char* bb = cur + (size_t)(b * 2 * NJ * 8 + b_off) * F::ELEM;
+            Pl xn[NPL];
+            if constexpr (bpref) {
+                if (b + 1 < NB) planes(b + 1, xn);
+            }
             // phasors of the next bin first: the operand formation below may
             // then overwrite p in place
             float2 pn[8];
@@ -336,6 +358,26 @@ void fdd_mfma_k(
                 const bf16x8 Arh = __builtin_bit_cast(bf16x8, prh), Arl = __builtin_bit_cast(bf16x8, prl);
                 const bf16x8 Aih = __builtin_bit_cast(bf16x8, pih), Ail = __builtin_bit_cast(bf16x8, pil);
                 f32x16 ar = acc_r[b], ai = acc_i[b];
+                if constexpr (bpref) {
+                    uint32_t nih[4], nil[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) nih[s] = pih[s] ^ 0x80008000u, nil[s] = pil[s] ^ 0x80008000u;
+                    const bf16x8 nAih = __builtin_bit_cast(bf16x8, nih), nAil = __builtin_bit_cast(bf16x8, nil);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, xc[0], ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, xc[0], ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, xc[0], ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ail, xc[0], ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, xc[1], ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aih, xc[1], ai, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, xc[2], ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAih, xc[2], ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arl, xc[2], ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAil, xc[2], ar, 0, 0, 0);
+                    ai = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, xc[3], ai, 0, 0, 0);
+                    ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nAih, xc[3], ar, 0, 0, 0);
+                    acc_r[b] = ar;
+                    acc_i[b] = ai;
+                } else {
                 // grouped by spectra plane (each read once, used, then dead)
                 bf16x8 x = X[0];  // re hi
                 ar = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Arh, x, ar, 0, 0, 0);
@@ -373,9 +415,10 @@ void fdd_mfma_k(
                 }
                 acc_r[b] = ar;
                 acc_i[b] = ai;
+                }
             } else {
                 const f16x8* X = reinterpret_cast<const f16x8*>(bb);
-                const f16x8 xr = X[0], xi = X[SLAB / 8];
+                const f16x8 xr = bpref ? xc[0] : X[0], xi = bpref ? xc[1] : X[SLAB / 8];
                 uint32_t pr[4], pi[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
@@ -394,6 +437,12 @@ void fdd_mfma_k(
                     acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, X[2 * SLAB / 8], acc_r[b], 0, 0, 0);
                 }
                 acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ai, xr, acc_i[b], 0, 0, 0);
+            }
+            if constexpr (bpref) {
+                if (b + 1 < NB) {
+#pragma unroll
+                    for (int t = 0; t < NPL; ++t) xc[t] = xn[t];
+                }
             }
             if (b + 1 < NB) {
 #pragma unroll
