@@ -74,6 +74,16 @@ class StreamStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """dedisp_amd_tuning (include/dedisp_amd.h)."""
+    _fields_ = [("samples_per_lane", C.c_uint), ("dms_per_wave", C.c_uint), ("waves", C.c_uint),
+                ("chunk_channels", C.c_uint), ("nfft", C.c_ulong), ("best_ms", C.c_double),
+                ("candidates", C.c_uint)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class FilHeader(C.Structure):
     _fields_ = [
         ("header_bytes", C.c_long), ("data_bytes", C.c_long),
@@ -94,6 +104,9 @@ _lib = None
 
 # (name, restype, argtypes) of every exported entry point of include/*.h
 _SIG = [
+    ("dedisp_amd_execute_batch", C.c_int, [C.c_void_p, C.c_ulong, C.POINTER(C.c_ulong), C.POINTER(C.c_void_p),
+                                           C.c_ulong, C.POINTER(C.c_void_p), C.c_ulong, C.c_uint]),
+    ("dedisp_amd_autotune", C.c_int, [C.c_void_p, C.c_ulong, C.c_ulong, C.c_ulong, C.c_void_p]),
     ("dedisp_select_implementation", C.c_int, [C.c_int]),
     ("dedisp_create_plan", C.c_int, [C.POINTER(C.c_void_p), C.c_ulong, C.c_float, C.c_float, C.c_float]),
     ("dedisp_destroy_plan", None, [C.c_void_p]),
@@ -346,6 +359,30 @@ class Plan:
 
     def fdd_set_segmented(self, enable: bool, nfft: int = 0) -> None:
         _check(lib().dedisp_amd_fdd_set_segmented(self._h, int(enable), nfft), "dedisp_amd_fdd_set_segmented")
+
+    def execute_batch_raw(self, nsamps, in_ptrs, in_nbits, out_ptrs, out_nbits, flags=0) -> int:
+        """dedisp_amd_execute_batch over len(nsamps) beams (host arrays or device tensors)."""
+        n = len(nsamps)
+        ns = (C.c_ulong * n)(*[int(x) for x in nsamps])
+        ip = (C.c_void_p * n)(*[_ptr(x).value for x in in_ptrs])
+        op = (C.c_void_p * n)(*[_ptr(x).value for x in out_ptrs])
+        return lib().dedisp_amd_execute_batch(self._h, n, ns, ip, in_nbits, op, out_nbits, flags)
+
+    def execute_batch(self, beams, in_nbits: int, out_nbits: int = 32, flags: int = 0) -> list:
+        """Host-pointer batch: beams is a list of uint8 arrays (time-major rows);
+        returns one [ndm, out_length] array per beam."""
+        beams = [np.ascontiguousarray(b) for b in beams]
+        row = self.channel_count * in_nbits // 8
+        nsamps = [b.nbytes // row for b in beams]
+        dt = {8: np.uint8, 16: np.uint16, 32: np.float32}[out_nbits]
+        outs = [np.zeros((self.dm_count, self.out_length(n)), dtype=dt) for n in nsamps]
+        _check(self.execute_batch_raw(nsamps, beams, in_nbits, outs, out_nbits, flags), "dedisp_amd_execute_batch")
+        return outs
+
+    def autotune(self, nsamps: int, in_nbits: int, out_nbits: int = 32) -> dict:
+        t = Tuning()
+        _check(lib().dedisp_amd_autotune(self._h, nsamps, in_nbits, out_nbits, C.byref(t)), "dedisp_amd_autotune")
+        return t.as_dict()
 
 
 def pack_samples(samples: np.ndarray, nbits: int) -> np.ndarray:

@@ -281,6 +281,34 @@ dedisp_error dedisp_execute(const dedisp_plan plan, dedisp_size nsamps, const de
     return guarded([&] { plan->ptr->execute(nsamps, in, in_nbits, out, out_nbits, flags); });
 }
 
+dedisp_error dedisp_amd_execute_batch(dedisp_plan plan, dedisp_size nbeams, const dedisp_size* nsamps,
+                                      const dedisp_byte* const* in, dedisp_size in_nbits,
+                                      dedisp_byte* const* out, dedisp_size out_nbits, unsigned flags)
+{
+    if (!plan) return DEDISP_INVALID_PLAN;
+    if (prior_gpu_error()) return DEDISP_PRIOR_GPU_ERROR;
+    if (nbeams && (!nsamps || !in || !out)) return DEDISP_INVALID_POINTER;
+    if (dedisp::TDDPlan* t = as_tdd(plan))
+        return guarded([&] { t->execute_batch(nbeams, nsamps, in, in_nbits, out, out_nbits, flags); });
+    // other plans: one beam after another
+    return guarded([&] {
+        for (dedisp_size b = 0; b < nbeams; ++b)
+            plan->ptr->execute(nsamps[b], in[b], in_nbits, out[b], out_nbits, flags);
+    });
+}
+
+dedisp_error dedisp_amd_autotune(dedisp_plan plan, dedisp_size nsamps, dedisp_size in_nbits,
+                                 dedisp_size out_nbits, dedisp_amd_tuning* result)
+{
+    if (!plan) return DEDISP_INVALID_PLAN;
+    if (prior_gpu_error()) return DEDISP_PRIOR_GPU_ERROR;
+    if (dedisp::TDDPlan* t = as_tdd(plan))
+        return guarded([&] { t->autotune(nsamps, in_nbits, out_nbits, result); });
+    if (dedisp::FDDGPUPlan* f = as_fdd(plan))
+        return guarded([&] { f->autotune(nsamps, in_nbits, out_nbits, result); });
+    return DEDISP_INVALID_PLAN;
+}
+
 dedisp_error dedisp_sync(void)
 {
     if (hipDeviceSynchronize() != hipSuccess) {

@@ -103,6 +103,67 @@ void FDDGPUPlan::set_segmented(int enable, dedisp_size nfft)
     m_state->nfft_req = nfft;
 }
 
+double FDDGPUPlan::autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
+                            dedisp_amd_tuning* chosen)
+{
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps <= m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    if (m_state->output != DEDISP_AMD_FDD_TIMESERIES) DD_THROW(DEDISP_INVALID_PLAN);
+    GPUContext& ctx = *m_ctx;
+    ctx.make_current();
+    const size_t row_bytes = m_nchans * in_nbits / 8, nsc = nsamps - m_max_delay;
+    DeviceBuffer d_in, d_out;
+    d_in.reserve(nsamps * row_bytes);
+    d_out.reserve(m_dm_count * nsc * 4);
+    DD_HIP(hipMemsetAsync(d_in.ptr, 0, nsamps * row_bytes, ctx.exec));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DD_HIP(hipEventCreate(&e0));
+    DD_HIP(hipEventCreate(&e1));
+    const int saved_seg = m_state->segmented;
+    const dedisp_size saved_nfft = m_state->nfft_req;
+    double best = -1;
+    dedisp_size best_n = 0;
+    unsigned timed = 0;
+    try {
+        for (dedisp_size n = 8192; n <= 65536; n *= 2) {
+            if (n <= m_max_delay || !dd_fdd_fft_supported(n)) continue;
+            set_segmented(1, n);
+            execute(nsamps, d_in.as<uint8_t>(), in_nbits, d_out.as<uint8_t>(), 32, DEDISP_DEVICE_POINTERS);
+            DD_HIP(hipEventRecord(e0, ctx.exec));
+            execute(nsamps, d_in.as<uint8_t>(), in_nbits, d_out.as<uint8_t>(), 32, DEDISP_DEVICE_POINTERS);
+            DD_HIP(hipEventRecord(e1, ctx.exec));
+            DD_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ++timed;
+            if (best < 0 || ms < best) best = ms, best_n = n;
+        }
+    } catch (...) {
+        m_state->segmented = saved_seg;
+        m_state->nfft_req = saved_nfft;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best >= 0) {
+        set_segmented(1, best_n);
+    } else {
+        m_state->segmented = saved_seg;
+        m_state->nfft_req = saved_nfft;
+    }
+    if (chosen) {
+        *chosen = dedisp_amd_tuning{};
+        chosen->nfft = best_n;
+        chosen->best_ms = best;
+        chosen->candidates = timed;
+    }
+    return best;
+}
+
 dedisp_size FDDGPUPlan::output_length(dedisp_size nsamps) const
 {
     const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");

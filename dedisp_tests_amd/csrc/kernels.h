@@ -60,6 +60,30 @@ hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_words, unsigne
 
 // Whether dd_tdd_dedisperse_lds has an instantiation for this tiling.
 int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tiling);
+// The instantiated tilings for nbits (16-channel chunks): writes up to max
+// into out and returns how many there are.
+int dd_tdd_lds_tilings(unsigned nbits, dd_tdd_tiling* out, int max);
+
+// One beam of a batched channel sum: its transposed rows, output rows and
+// the index of its first time tile among all beams' tiles.
+typedef struct {
+    const uint32_t* rows;   // [nchans][row_words]
+    uint8_t* out;           // DM row i at out + i*out_stride
+    uint64_t row_words;
+    uint64_t out_stride;    // bytes
+    uint32_t nsamps_out;    // > 0
+    uint32_t tile0;
+} dd_tdd_beam;
+
+// Packed-lane channel sum over several beams in one launch (same DM list,
+// channels, tables and tiling). beams: [nbeams] on the device; tile_beam:
+// [ntiles] beam index of every time tile (device), ntiles = sum over beams of
+// ceil(nsamps_out / (64 * samples_per_lane)).
+hipError_t dd_tdd_dedisperse_lds_batch(const dd_tdd_beam* beams, const uint32_t* tile_beam,
+                                       size_t ntiles, unsigned nbits, const uint32_t* code,
+                                       const int32_t* chan, const uint32_t* wstart, size_t nact,
+                                       size_t ndm, dd_tdd_epilogue ep, dd_tdd_tiling tiling,
+                                       hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -167,7 +167,8 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     const uint32_t* __restrict__ code, const int32_t* __restrict__ chan,
     const uint32_t* __restrict__ wstart, uint32_t nact, uint32_t ndm,
     uint32_t nsamps_out, uint32_t ndmblk, uint32_t cc_max, uint32_t W,
-    uint8_t* __restrict__ out, size_t out_stride, dd_tdd_epilogue ep)
+    uint8_t* __restrict__ out, size_t out_stride, dd_tdd_epilogue ep,
+    const dd_tdd_beam* __restrict__ beams, const uint32_t* __restrict__ tile_beam)
 {
     constexpr int SPW = 32 / NBITS;
     constexpr int P = (NBITS < 8) ? 8 / NBITS : 1;
@@ -186,7 +187,18 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     const uint32_t nb = gridDim.x, bid = blockIdx.x;
     const uint32_t xcd = bid & 7u, per = nb >> 3, rem = nb & 7u;
     const uint32_t L = xcd * per + min(xcd, rem) + (bid >> 3);
-    const uint32_t tile = L / ndmblk, dblk = L % ndmblk;
+    const uint32_t tile_g = L / ndmblk, dblk = L % ndmblk;
+    // batched launch: the time tile's beam supplies rows, output and length
+    uint32_t tile = tile_g;
+    if (beams) {
+        const dd_tdd_beam bm = beams[tile_beam[tile_g]];
+        tile = tile_g - bm.tile0;
+        rows = bm.rows;
+        row_words = bm.row_words;
+        nsamps_out = bm.nsamps_out;
+        out = bm.out;
+        out_stride = bm.out_stride;
+    }
 
     const int lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -409,11 +421,13 @@ template <int NBITS, int G, int DPW, int WAVES, bool NEED32>
 hipError_t launch_lds(const uint32_t* rows, size_t row_words, const uint32_t* code,
                       const int32_t* chan, const uint32_t* wstart, size_t nact, size_t ndm,
                       size_t nsamps_out, uint8_t* out, size_t out_stride,
-                      dd_tdd_epilogue ep, const dd_tdd_tiling& tl, hipStream_t stream)
+                      dd_tdd_epilogue ep, const dd_tdd_tiling& tl, hipStream_t stream,
+                      const dd_tdd_beam* beams = nullptr, const uint32_t* tile_beam = nullptr,
+                      size_t batch_tiles = 0)
 {
     constexpr int SPW = 32 / NBITS;
     const size_t tile_samps = (size_t)64 * G * SPW;
-    const size_t ntiles = (nsamps_out + tile_samps - 1) / tile_samps;
+    const size_t ntiles = beams ? batch_tiles : (nsamps_out + tile_samps - 1) / tile_samps;
     const size_t ndmblk = (ndm + WAVES * DPW - 1) / (WAVES * DPW);
     const size_t nblocks = ntiles * ndmblk;
     const size_t lds_bytes = (size_t)tl.chunk_channels * tl.window_words * 4;
@@ -427,7 +441,7 @@ hipError_t launch_lds(const uint32_t* rows, size_t row_words, const uint32_t* co
     hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(WAVES * 64), lds_bytes, stream,
                        rows, row_words, code, chan, wstart, (uint32_t)nact, (uint32_t)ndm,
                        (uint32_t)nsamps_out, (uint32_t)ndmblk, tl.chunk_channels,
-                       tl.window_words, out, out_stride, ep);
+                       tl.window_words, out, out_stride, ep, beams, tile_beam);
     return hipGetLastError();
 }
 
@@ -487,6 +501,21 @@ extern "C" hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row
     X(4, 2, 1, 4) X(4, 1, 2, 8) \
     X(8, 2, 2, 4) X(8, 4, 2, 4) X(8, 2, 4, 4) X(8, 2, 2, 8)
 
+extern "C" int dd_tdd_lds_tilings(unsigned nbits, dd_tdd_tiling* out, int max)
+{
+    int n = 0;
+#define DD_LIST(NB, GG, DPW, WAVES)                                        \
+    if (nbits == NB) {                                                     \
+        if (n < max && out) {                                              \
+            out[n] = dd_tdd_tiling{(unsigned)(GG * (32 / NB)), DPW, WAVES, 16, 0}; \
+        }                                                                  \
+        ++n;                                                               \
+    }
+    DD_LDS_TILINGS(DD_LIST)
+#undef DD_LIST
+    return n;
+}
+
 extern "C" int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tl)
 {
     const unsigned spw = 32 / nbits;
@@ -521,6 +550,33 @@ extern "C" hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_wor
         return launch_lds<NB, GG, DPW, WAVES, false>(rows, row_words, code, chan, wstart, nact, \
                                                      ndm, nsamps_out, out, out_stride, ep, tl,  \
                                                      stream);                                   \
+    }
+    DD_LDS_TILINGS(DD_CALL)
+#undef DD_CALL
+    return hipErrorInvalidValue;
+}
+
+extern "C" hipError_t dd_tdd_dedisperse_lds_batch(const dd_tdd_beam* beams, const uint32_t* tile_beam,
+                                                  size_t ntiles, unsigned nbits, const uint32_t* code,
+                                                  const int32_t* chan, const uint32_t* wstart, size_t nact,
+                                                  size_t ndm, dd_tdd_epilogue ep, dd_tdd_tiling tl,
+                                                  hipStream_t stream)
+{
+    if (ntiles == 0 || ndm == 0) return hipSuccess;
+    if (!beams || !tile_beam) return hipErrorInvalidValue;
+    const unsigned spw = 32 / nbits;
+    const unsigned G = tl.samples_per_lane / spw;
+    const uint32_t maxv = (1u << nbits) - 1u;
+    const bool need32 = (uint64_t)nact * maxv > 65535u;
+#define DD_CALL(NB, GG, DPW, WAVES)                                                              \
+    if (nbits == NB && G == GG && tl.dms_per_wave == DPW && tl.waves == WAVES) {                 \
+        if (need32)                                                                              \
+            return launch_lds<NB, GG, DPW, WAVES, true>(nullptr, 0, code, chan, wstart, nact,    \
+                                                        ndm, 0, nullptr, 0, ep, tl, stream,      \
+                                                        beams, tile_beam, ntiles);               \
+        return launch_lds<NB, GG, DPW, WAVES, false>(nullptr, 0, code, chan, wstart, nact, ndm,  \
+                                                     0, nullptr, 0, ep, tl, stream, beams,       \
+                                                     tile_beam, ntiles);                         \
     }
     DD_LDS_TILINGS(DD_CALL)
 #undef DD_CALL

@@ -29,7 +29,7 @@ struct TDDState {
     // cache key of the device tables
     unsigned long version = ~0ul;
     unsigned nbits = 0;
-    size_t first_dm = ~size_t(0), dm_count = 0, nsamps_out = ~size_t(0);
+    size_t first_dm = ~size_t(0), dm_count = 0;
 
     // derived from the tables
     bool fast = false;
@@ -38,6 +38,29 @@ struct TDDState {
     size_t dmax = 0;        // largest delay (samples) over the DMs in use
     size_t max_wstart = 0;  // fast path
     size_t row_words = 0;
+
+    // tilings chosen by the autotuner, per input width (waves == 0: none)
+    dd_tdd_tiling tuned[33] = {};
+    dd_tdd_tiling tiling_key{};
+
+    // batched execution (own buffers, so a cached graph's pointers stay valid
+    // whatever single executes do in between)
+    DeviceBuffer d_brows, d_bin, d_bout, d_beams, d_tile_beam;
+    std::vector<dd_tdd_beam> h_beams;
+    std::vector<uint32_t> h_tile_beam;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+
+    void drop_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
+    ~TDDState() { drop_graph(); }
 };
 
 namespace {
@@ -82,67 +105,27 @@ TDDPlan::~TDDPlan() {}
 
 void TDDPlan::set_gulp_size(size_type gulp_size) { m_gulp_size = gulp_size; }
 
-void TDDPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
-                      byte_type* out, size_type out_nbits, unsigned flags)
+// Device tables for input width nbits and DMs first_dm_idx .. +dm_count:
+// active channels, integer delays, and for the packed-lane kernel the window
+// starts and delay codes. Cached per DM list / killmask / width / DM range /
+// tiling; independent of the series length.
+void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count)
 {
-    // src/tdd/TDDPlan.cpp:102-127: unpadded strides
-    dedisp_size out_bytes_per_sample = out_nbits / 8;
-    dedisp_size in_stride = m_nchans * in_nbits / 8;
-    dedisp_size out_stride = (nsamps - m_max_delay) * out_bytes_per_sample;
-    execute_adv(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, flags);
-}
-
-void TDDPlan::execute_adv(size_type nsamps, const byte_type* in, size_type in_nbits,
-                          size_type in_stride, byte_type* out, size_type out_nbits,
-                          size_type out_stride, unsigned flags)
-{
-    execute_guru(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, 0, m_dm_count, flags);
-}
-
-void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_nbits,
-                           size_type in_stride, byte_type* out, size_type out_nbits,
-                           size_type out_stride, dedisp_size first_dm_idx,
-                           dedisp_size dm_count, unsigned flags)
-{
-    // Validation in the order of src/tdd/TDDPlan.cpp:167-195 (the unsigned
-    // wrap of nsamps - max_delay in the stride test is kept, so nsamps <
-    // max_delay reports DEDISP_INVALID_STRIDE as the reference does).
-    dedisp_size out_bytes_per_sample = out_nbits / 8;
-    if (!in || !out) DD_THROW(DEDISP_INVALID_POINTER);
-    if (in_stride < m_nchans * in_nbits / 8 ||
-        out_stride < (nsamps - m_max_delay) * out_bytes_per_sample)
-        DD_THROW(DEDISP_INVALID_STRIDE);
-    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
-    if (nsamps < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
-    // src/dedisp/DedispPlan.cpp:137-139
-    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
-    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
-        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
-    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8 && in_nbits != 16 &&
-        in_nbits != 32)
-        DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
-    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
-    if (first_dm_idx > m_dm_count || dm_count > m_dm_count - first_dm_idx)
-        DD_THROW(DEDISP_UNKNOWN_ERROR);
-    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
-
-    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
-    const size_t nsc = nsamps - m_max_delay;
-    if (nsc == 0 || dm_count == 0) return;
-
     GPUContext& ctx = *m_ctx;
     TDDState& st = *m_state;
-    ctx.make_current();
     hipStream_t stream = ctx.exec;
-    ctx.timer.reset(ctx.profiling);
-    ctx.profile = dedisp_amd_profile{};
-
-    const unsigned nbits = (unsigned)in_nbits;
     const size_t spw = 32 / nbits;
-
-    // ---- device tables (cached per DM list / killmask / nbits / DM range) ----
+    dd_tdd_tiling want = default_tiling(nbits);
+    if (st.tuned[nbits].waves) want = st.tuned[nbits];
+    if (const char* env = std::getenv("DEDISP_TDD_TILING")) {
+        unsigned a, b, c, e;
+        if (std::sscanf(env, "%u,%u,%u,%u", &a, &b, &c, &e) == 4) {
+            want.samples_per_lane = a; want.dms_per_wave = b; want.waves = c; want.chunk_channels = e;
+        }
+    }
     if (st.version != m_tables_version || st.nbits != nbits || st.first_dm != first_dm_idx ||
-        st.dm_count != dm_count || st.nsamps_out != nsc) {
+        st.dm_count != dm_count || std::memcmp(&st.tiling_key, &want, sizeof want) != 0) {
+        st.tiling_key = want;
         std::vector<int32_t> chan, weight;
         bool binary = true;
         for (size_t c = 0; c < m_nchans; ++c) {
@@ -164,13 +147,7 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
             }
         }
 
-        dd_tdd_tiling tl = default_tiling(nbits);
-        if (const char* env = std::getenv("DEDISP_TDD_TILING")) {
-            unsigned a, b, c, e;
-            if (std::sscanf(env, "%u,%u,%u,%u", &a, &b, &c, &e) == 4) {
-                tl.samples_per_lane = a; tl.dms_per_wave = b; tl.waves = c; tl.chunk_channels = e;
-            }
-        }
+        dd_tdd_tiling tl = want;
         bool fast = binary && nact > 0 && nbits <= 8 && dd_tdd_lds_supported(nbits, tl) &&
                     (uint64_t)dm_count * nact < (uint64_t(1) << 31);
 
@@ -228,17 +205,6 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
             }
         }
 
-        size_t row_words;
-        if (fast) {
-            const size_t tile_words = (size_t)64 * (tl.samples_per_lane / spw);
-            const size_t ntiles = div_up(nsc, tile_words * spw);
-            row_words = ntiles * tile_words + max_wstart + W;
-        } else {
-            row_words = (nsc + dmax) / spw + 2;
-        }
-        row_words = std::max(row_words, div_up(nsamps, spw));
-        row_words = round_up(row_words, 64);
-
         st.d_chan.reserve(std::max<size_t>(nact, 1) * 4);
         st.d_weight.reserve(std::max<size_t>(nact, 1) * 4);
         if (nact) {
@@ -260,14 +226,91 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
         st.nbits = nbits;
         st.first_dm = first_dm_idx;
         st.dm_count = dm_count;
-        st.nsamps_out = nsc;
         st.fast = fast;
         st.tiling = tl;
         st.nact = nact;
         st.dmax = dmax;
         st.max_wstart = max_wstart;
-        st.row_words = row_words;
     }
+
+}
+
+// Words per channel row of the transposed input for nsamps samples with nsc
+// output samples: every (DM, output) window of every time tile lies inside.
+size_t TDDPlan::row_words_for(unsigned nbits, size_t nsamps, size_t nsc) const
+{
+    const TDDState& st = *m_state;
+    const size_t spw = 32 / nbits;
+    size_t row_words;
+    if (st.fast) {
+        const size_t tile_words = (size_t)64 * (st.tiling.samples_per_lane / spw);
+        const size_t ntiles = div_up(nsc, tile_words * spw);
+        row_words = ntiles * tile_words + st.max_wstart + st.tiling.window_words;
+    } else {
+        row_words = (nsc + st.dmax) / spw + 2;
+    }
+    row_words = std::max(row_words, div_up(nsamps, spw));
+    return round_up(row_words, 64);
+}
+
+void TDDPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
+                      byte_type* out, size_type out_nbits, unsigned flags)
+{
+    // src/tdd/TDDPlan.cpp:102-127: unpadded strides
+    dedisp_size out_bytes_per_sample = out_nbits / 8;
+    dedisp_size in_stride = m_nchans * in_nbits / 8;
+    dedisp_size out_stride = (nsamps - m_max_delay) * out_bytes_per_sample;
+    execute_adv(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, flags);
+}
+
+void TDDPlan::execute_adv(size_type nsamps, const byte_type* in, size_type in_nbits,
+                          size_type in_stride, byte_type* out, size_type out_nbits,
+                          size_type out_stride, unsigned flags)
+{
+    execute_guru(nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, 0, m_dm_count, flags);
+}
+
+void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_nbits,
+                           size_type in_stride, byte_type* out, size_type out_nbits,
+                           size_type out_stride, dedisp_size first_dm_idx,
+                           dedisp_size dm_count, unsigned flags)
+{
+    // Validation in the order of src/tdd/TDDPlan.cpp:167-195 (the unsigned
+    // wrap of nsamps - max_delay in the stride test is kept, so nsamps <
+    // max_delay reports DEDISP_INVALID_STRIDE as the reference does).
+    dedisp_size out_bytes_per_sample = out_nbits / 8;
+    if (!in || !out) DD_THROW(DEDISP_INVALID_POINTER);
+    if (in_stride < m_nchans * in_nbits / 8 ||
+        out_stride < (nsamps - m_max_delay) * out_bytes_per_sample)
+        DD_THROW(DEDISP_INVALID_STRIDE);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    // src/dedisp/DedispPlan.cpp:137-139
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8 && in_nbits != 16 &&
+        in_nbits != 32)
+        DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    if (first_dm_idx > m_dm_count || dm_count > m_dm_count - first_dm_idx)
+        DD_THROW(DEDISP_UNKNOWN_ERROR);
+    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const size_t nsc = nsamps - m_max_delay;
+    if (nsc == 0 || dm_count == 0) return;
+
+    GPUContext& ctx = *m_ctx;
+    TDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    ctx.timer.reset(ctx.profiling);
+    ctx.profile = dedisp_amd_profile{};
+
+    const unsigned nbits = (unsigned)in_nbits;
+    prepare_tables(nbits, first_dm_idx, dm_count);
+    st.row_words = row_words_for(nbits, nsamps, nsc);
 
     // ---- input ----
     const size_t row_bytes = (m_nchans * nbits + 7) / 8;
@@ -338,6 +381,250 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
         ctx.timer.collect(ctx.profile);
         ctx.profile.work_units = (double)dm_count * (double)nsc * (double)m_nchans;
     }
+}
+
+namespace {
+
+dd_tdd_epilogue make_epilogue(unsigned nbits, size_t out_nbits, size_t nchans)
+{
+    // scale_output constants (src/tdd/dedisperse/dedisperse_kernel.cuh:50-58)
+    dd_tdd_epilogue ep;
+    uint32_t in_max = (nbits == 32) ? 0xffffffffu : ((1u << nbits) - 1u);
+    float in_range = (float)in_max;
+    ep.out_range = (out_nbits == 32) ? 1.f : (float)((out_nbits == 16) ? 65535u : 255u);
+    ep.den = in_range * (float)nchans;
+    ep.factor = (3.f * 1024.f) / 255.f / 16.f;
+    ep.out_nbits = (unsigned)out_nbits;
+    return ep;
+}
+
+}  // namespace
+
+void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byte_type* const* in,
+                            size_type in_nbits, byte_type* const* out, size_type out_nbits,
+                            unsigned flags)
+{
+    if (nbeams == 0) return;
+    if (!nsamps || !in || !out) DD_THROW(DEDISP_INVALID_POINTER);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8 && in_nbits != 16 &&
+        in_nbits != 32)
+        DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    for (size_t b = 0; b < nbeams; ++b) {
+        if (!in[b] || !out[b]) DD_THROW(DEDISP_INVALID_POINTER);
+        if (nsamps[b] < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+        if (nsamps[b] >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    }
+    const unsigned nbits = (unsigned)in_nbits;
+    const size_t row_bytes = (m_nchans * nbits + 7) / 8;
+    const size_t out_bytes = out_nbits / 8;
+
+    GPUContext& ctx = *m_ctx;
+    TDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    prepare_tables(nbits, 0, m_dm_count);
+    if (!st.fast || st.nact == 0) {
+        // weighted killmask or 16/32-bit samples: one beam after another
+        for (size_t b = 0; b < nbeams; ++b)
+            execute_guru(nsamps[b], in[b], in_nbits, row_bytes, out[b], out_nbits,
+                         (nsamps[b] - m_max_delay) * out_bytes, 0, m_dm_count, flags);
+        return;
+    }
+    ctx.timer.reset(ctx.profiling);
+    ctx.profile = dedisp_amd_profile{};
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+
+    // ---- layout: rows, staged input/output and time tiles of every beam ----
+    const size_t tile_samps = (size_t)64 * st.tiling.samples_per_lane;
+    std::vector<size_t> nsc(nbeams), rw(nbeams), roff(nbeams), ioff(nbeams), ooff(nbeams), nt(nbeams);
+    size_t rows_total = 0, in_total = 0, out_total = 0, ntiles = 0;
+    double work = 0;
+    for (size_t b = 0; b < nbeams; ++b) {
+        nsc[b] = nsamps[b] - m_max_delay;
+        rw[b] = row_words_for(nbits, nsamps[b], nsc[b]);
+        roff[b] = rows_total;
+        rows_total += m_nchans * rw[b];
+        ioff[b] = in_total;
+        in_total += round_up(nsamps[b] * row_bytes, 256);
+        ooff[b] = out_total;
+        out_total += round_up(m_dm_count * nsc[b] * out_bytes, 256);
+        nt[b] = div_up(nsc[b], tile_samps);
+        ntiles += nt[b];
+        work += (double)m_dm_count * (double)nsc[b] * (double)m_nchans;
+    }
+    if (ntiles == 0) return;
+    st.d_brows.reserve(rows_total * 4);
+    if (!device_ptrs) {
+        st.d_bin.reserve(in_total);
+        st.d_bout.reserve(out_total);
+    }
+    st.d_beams.reserve(nbeams * sizeof(dd_tdd_beam));
+    st.d_tile_beam.reserve(ntiles * 4);
+    uint8_t* const bout = device_ptrs ? nullptr : st.d_bout.as<uint8_t>();
+    const uint8_t* const bin = device_ptrs ? nullptr : st.d_bin.as<uint8_t>();
+
+    std::vector<uint64_t> key = {nbeams, nbits, out_nbits, m_tables_version,
+                                 st.tiling.samples_per_lane, st.tiling.dms_per_wave, st.tiling.waves,
+                                 st.tiling.chunk_channels, st.tiling.window_words,
+                                 (uint64_t)st.d_code.ptr, (uint64_t)st.d_chan.ptr, (uint64_t)st.d_wstart.ptr,
+                                 (uint64_t)st.d_brows.ptr, (uint64_t)st.d_beams.ptr, (uint64_t)st.d_tile_beam.ptr};
+    for (size_t b = 0; b < nbeams; ++b) {
+        key.push_back(nsamps[b]);
+        key.push_back((uint64_t)in[b]);
+        key.push_back((uint64_t)out[b]);
+    }
+    const dd_tdd_epilogue ep = make_epilogue(nbits, out_nbits, m_nchans);
+    const bool replay = device_ptrs && st.graph_exec && st.graph_key == key;
+
+    if (!replay) {
+        st.h_beams.resize(nbeams);
+        st.h_tile_beam.clear();
+        st.h_tile_beam.reserve(ntiles);
+        size_t t0 = 0;
+        for (size_t b = 0; b < nbeams; ++b) {
+            dd_tdd_beam& e = st.h_beams[b];
+            e.rows = st.d_brows.as<uint32_t>() + roff[b];
+            e.out = device_ptrs ? out[b] : bout + ooff[b];
+            e.row_words = rw[b];
+            e.out_stride = nsc[b] * out_bytes;
+            e.nsamps_out = (uint32_t)nsc[b];
+            e.tile0 = (uint32_t)t0;
+            st.h_tile_beam.insert(st.h_tile_beam.end(), nt[b], (uint32_t)b);
+            t0 += nt[b];
+        }
+        DD_HIP(hipMemcpyAsync(st.d_beams.ptr, st.h_beams.data(), nbeams * sizeof(dd_tdd_beam),
+                              hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_tile_beam.ptr, st.h_tile_beam.data(), ntiles * 4,
+                              hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+    }
+
+    // transposes of every beam, then one channel-sum launch over all tiles
+    auto launches = [&](bool timed) {
+        if (timed) ctx.timer.begin(StageTimer::UNPACK, stream);
+        for (size_t b = 0; b < nbeams; ++b) {
+            const uint8_t* src = device_ptrs ? in[b] : bin + ioff[b];
+            DD_HIP(dd_tdd_transpose(src, nsamps[b], row_bytes, nbits, m_nchans,
+                                    st.d_brows.as<uint32_t>() + roff[b], rw[b], stream));
+        }
+        if (timed) ctx.timer.end(StageTimer::UNPACK, stream);
+        if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+        DD_HIP(dd_tdd_dedisperse_lds_batch(st.d_beams.as<dd_tdd_beam>(), st.d_tile_beam.as<uint32_t>(),
+                                           ntiles, nbits, st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
+                                           st.d_wstart.as<uint32_t>(), st.nact, m_dm_count, ep,
+                                           st.tiling, stream));
+        if (timed) ctx.timer.end(StageTimer::DEDISPERSE, stream);
+    };
+
+    if (device_ptrs) {
+        if (!replay) {
+            // capture the kernels once (no copies or fills inside the graph)
+            st.drop_graph();
+            DD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launches(false);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            DD_HIP(hipStreamEndCapture(stream, &st.graph));
+            DD_HIP(hipGraphInstantiate(&st.graph_exec, st.graph, nullptr, nullptr, 0));
+            st.graph_key = key;
+        }
+        ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+        DD_HIP(hipGraphLaunch(st.graph_exec, stream));
+        ctx.timer.end(StageTimer::DEDISPERSE, stream);
+    } else {
+        ctx.timer.begin(StageTimer::H2D, stream);
+        for (size_t b = 0; b < nbeams; ++b)
+            DD_HIP(hipMemcpyAsync(st.d_bin.as<uint8_t>() + ioff[b], in[b], nsamps[b] * row_bytes,
+                                  hipMemcpyHostToDevice, stream));
+        ctx.timer.end(StageTimer::H2D, stream);
+        launches(true);
+        ctx.timer.begin(StageTimer::D2H, stream);
+        for (size_t b = 0; b < nbeams; ++b)
+            DD_HIP(hipMemcpyAsync(out[b], bout + ooff[b], m_dm_count * nsc[b] * out_bytes,
+                                  hipMemcpyDeviceToHost, stream));
+        ctx.timer.end(StageTimer::D2H, stream);
+    }
+    if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
+        DD_HIP(hipStreamSynchronize(stream));
+        ctx.timer.collect(ctx.profile);
+        ctx.profile.work_units = work;
+    }
+}
+
+double TDDPlan::autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
+                         dedisp_amd_tuning* chosen)
+{
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps <= m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    const unsigned nbits = (unsigned)in_nbits;
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    GPUContext& ctx = *m_ctx;
+    TDDState& st = *m_state;
+    ctx.make_current();
+    dd_tdd_tiling cand[64];
+    const int ncand = std::min(dd_tdd_lds_tilings(nbits, cand, 64), 64);
+    const size_t row_bytes = (m_nchans * nbits + 7) / 8;
+    const size_t nsc = nsamps - m_max_delay, out_bytes = out_nbits / 8;
+    DeviceBuffer d_in, d_out;
+    d_in.reserve(nsamps * row_bytes);
+    d_out.reserve(m_dm_count * nsc * out_bytes);
+    DD_HIP(hipMemsetAsync(d_in.ptr, 0, nsamps * row_bytes, ctx.exec));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DD_HIP(hipEventCreate(&e0));
+    DD_HIP(hipEventCreate(&e1));
+    const dd_tdd_tiling saved = st.tuned[nbits];
+    double best = -1;
+    dd_tdd_tiling best_t{};
+    int timed = 0;
+    try {
+        for (int i = 0; i < ncand; ++i) {
+            st.tuned[nbits] = cand[i];
+            auto run = [&] {
+                execute_guru(nsamps, d_in.as<uint8_t>(), in_nbits, row_bytes, d_out.as<uint8_t>(), out_nbits,
+                             nsc * out_bytes, 0, m_dm_count, DEDISP_DEVICE_POINTERS);
+            };
+            run();  // builds the tables for this tiling
+            if (!st.fast) break;  // weighted killmask: nothing to choose
+            DD_HIP(hipEventRecord(e0, ctx.exec));
+            run();
+            DD_HIP(hipEventRecord(e1, ctx.exec));
+            DD_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ++timed;
+            if (best < 0 || ms < best) best = ms, best_t = cand[i];
+        }
+    } catch (...) {
+        st.tuned[nbits] = saved;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.tuned[nbits] = (best >= 0) ? best_t : saved;
+    if (chosen) {
+        const dd_tdd_tiling t = (best >= 0) ? best_t : default_tiling(nbits);
+        *chosen = dedisp_amd_tuning{};
+        chosen->samples_per_lane = t.samples_per_lane;
+        chosen->dms_per_wave = t.dms_per_wave;
+        chosen->waves = t.waves;
+        chosen->chunk_channels = t.chunk_channels;
+        chosen->best_ms = best;
+        chosen->candidates = (unsigned)timed;
+    }
+    return best;
 }
 
 }  // namespace dedisp

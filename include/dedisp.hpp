@@ -147,7 +147,28 @@ public:
                       size_type out_stride, dedisp_size first_dm_idx,
                       dedisp_size dm_count, unsigned flags = 0);
 
+    /* Batched execution (extension, dedisp_amd_execute_batch): nbeams series
+     * of uneven length through the same DM list; beam b has nsamps[b]
+     * time-major samples at in[b] and gets (nsamps[b] - max_delay) outputs per
+     * DM at out[b] (DM-major, unpadded). With the packed-lane kernel one
+     * channel-sum launch covers every beam; with DEDISP_DEVICE_POINTERS the
+     * launches are captured once in a HIP graph and replayed while the
+     * batch (lengths, pointers, widths, tables) stays the same. */
+    void execute_batch(size_type nbeams, const size_type* nsamps, const byte_type* const* in,
+                       size_type in_nbits, byte_type* const* out, size_type out_nbits,
+                       unsigned flags = 0);
+
+    /* Times every instantiated tiling of the channel-sum kernel for this
+     * plan's DM list on nsamps samples of in_nbits input (device scratch) and
+     * keeps the fastest for later executes at that width. Returns the time of
+     * the chosen tiling in ms and the tiling in *chosen. */
+    double autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
+                    dedisp_amd_tuning* chosen);
+
 private:
+    void prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count);
+    size_t row_words_for(unsigned nbits, size_t nsamps, size_t nsc) const;
+
     dedisp_size m_gulp_size;
     std::unique_ptr<TDDState> m_state;
 };
@@ -179,6 +200,13 @@ public:
 
     /* Output length per DM, in floats, for nsamps input samples. */
     dedisp_size output_length(dedisp_size nsamps) const;
+
+    /* Times the overlap-save segment lengths 2^13 .. 2^16 longer than
+     * max_delay on nsamps samples of in_nbits input (device scratch,
+     * time-series output) and keeps the fastest (set_segmented(1, nfft)).
+     * Returns its time in ms and the choice in *chosen. */
+    double autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
+                    dedisp_amd_tuning* chosen);
 
 private:
     void execute_fdd(size_type nsamps, const byte_type* in, size_type in_nbits, byte_type* out,

@@ -165,6 +165,45 @@ dedisp_error dedisp_amd_write_dat(const char* prefix, dedisp_float dm, const ded
 dedisp_error dedisp_amd_write_inf(const char* prefix, const dedisp_amd_fil_header* header,
                                   dedisp_float dm, dedisp_size numout);
 
+/* ---------------------------------------------------------------------------
+ * Batched beams and autotuning (no reference counterpart: the reference runs
+ * one series per dedisp_execute and fixes its tile sizes at compile time).
+ * --------------------------------------------------------------------------- */
+
+/* nbeams series of uneven length through the plan's DM list: beam b has
+ * nsamps[b] time-major samples of in_nbits at in[b] (unpadded rows of
+ * nchans*in_nbits/8 bytes) and receives (nsamps[b] - max_delay) output
+ * samples per DM at out[b] (DM-major, unpadded rows), exactly as dedisp_execute
+ * on that beam alone would write them (FDD plans: their configured output).
+ * flags as for dedisp_execute; with DEDISP_DEVICE_POINTERS every in[b] and
+ * out[b] is device memory. TDD and legacy plans with a 0/1 killmask and
+ * in_nbits <= 8 transpose each beam and run one channel-sum launch over all
+ * beams; with DEDISP_DEVICE_POINTERS that sequence is captured in a HIP graph
+ * and replayed while lengths, pointers, widths and DM list stay the same.
+ * Other cases (and FDD plans) run the beams one after another. */
+dedisp_error dedisp_amd_execute_batch(dedisp_plan plan, dedisp_size nbeams,
+                                      const dedisp_size* nsamps, const dedisp_byte* const* in,
+                                      dedisp_size in_nbits, dedisp_byte* const* out,
+                                      dedisp_size out_nbits, unsigned flags);
+
+typedef struct {
+    /* TDD / legacy: tiling of the channel-sum kernel */
+    unsigned    samples_per_lane, dms_per_wave, waves, chunk_channels;
+    /* FDD: overlap-save segment length (time-series output) */
+    dedisp_size nfft;
+    double      best_ms;      /* execute time of the chosen configuration */
+    unsigned    candidates;   /* configurations timed */
+} dedisp_amd_tuning;
+
+/* Times the plan's configurations on nsamps samples of in_nbits input in
+ * device scratch (contents irrelevant to the timing) and keeps the fastest for
+ * later executes: TDD / legacy plans every instantiated channel-sum tiling for
+ * that width, FDD plans the overlap-save segment lengths 2^k >= 2 max_delay
+ * (2^13 .. 2^18, time-series output). Requires the DM list; returns the
+ * choice in *result. */
+dedisp_error dedisp_amd_autotune(dedisp_plan plan, dedisp_size nsamps, dedisp_size in_nbits,
+                                 dedisp_size out_nbits, dedisp_amd_tuning* result);
+
 #ifdef __cplusplus
 }
 #endif
