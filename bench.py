@@ -226,6 +226,84 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16")) 
     return out
 
 
+def batch_leg(torch, dd, steps: int = 2) -> dict:
+    """Batched beams: 64 beams of log-uniform length 2^17..2^20 (seeded), 512
+    channels from 1500 MHz down in 0.78125 MHz steps, dt 81.92 us, 2-bit
+    samples clip(round(N(1.5, 0.8)), 0, 3), 1024 linear DM trials 0..2000. TDD
+    with the autotuned tiling: one transpose per beam and one channel-sum
+    launch for all beams, replayed as one HIP graph (device pointers, 8-bit
+    output); then FDD-BF16 on the first 8 beams (beam by beam, float output).
+    Inputs and outputs stay in HBM."""
+    nb, nch, dt, f0, df = 64, 512, 81.92e-6, 1500.0, -0.78125
+    rng = np.random.default_rng(4)
+    lengths = [int(2 ** rng.uniform(17, 20)) for _ in range(nb)]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4)
+    d_in = []
+    for n in lengths:
+        x = torch.clamp(torch.round(torch.randn((n, nch), generator=g, device="cuda") * 0.8 + 1.5), 0, 3).to(torch.uint8)
+        d_in.append((x[:, 0::4] | (x[:, 1::4] << 2) | (x[:, 2::4] << 4) | (x[:, 3::4] << 6)).contiguous())
+        del x
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.TDD)
+    dms = np.linspace(0.0, 2000.0, 1024, dtype=np.float32)
+    plan.set_dm_list(dms)
+    md = plan.max_delay
+    tune = plan.autotune(1 << 20, 2, 8)
+    d_out = [torch.empty((1024, n - md), dtype=torch.uint8, device="cuda") for n in lengths]
+    torch.cuda.synchronize()
+
+    def step():
+        rc = plan.execute_batch_raw(lengths, d_in, 2, d_out, 8, dd.DEVICE_POINTERS)
+        if rc != 0:
+            raise dd.DedispError(rc, "dedisp_amd_execute_batch")
+
+    step()  # captures the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    dm_samples = 1024 * sum(n - md for n in lengths)
+    # parity spot check: the shortest beam equals its single execute
+    i = int(np.argmin(lengths))
+    one = torch.empty_like(d_out[i])
+    assert plan.execute_raw(lengths[i], d_in[i], 2, one, 8, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    parity = "ok" if torch.equal(one, d_out[i]) else "MISMATCH"
+    out = {"workload": "batch-64beam-512ch-2^17..2^20x2bit-1024dm", "beams": nb,
+           "total_samples": int(sum(lengths)), "max_delay": md, "tdd_tiling": tune,
+           "tdd": {"value": dm_samples / el, "unit": "DM*samples/s", "ms_per_batch": 1e3 * el,
+                   "launch": "one HIP graph: 64 transposes + 1 channel-sum launch", "out_nbits": 8,
+                   "parity": parity}}
+    del d_out, one, plan
+    torch.cuda.empty_cache()
+
+    # FDD-BF16 on the first 8 beams, one after another
+    sub = list(range(8))
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    plan.fdd_set_precision(dd.FDD_BF16)
+    plan.set_dm_list(dms)
+    ftune = plan.autotune(1 << 20, 2, 32)
+    f_out = [torch.empty((1024, lengths[b] - md), dtype=torch.float32, device="cuda") for b in sub]
+    rc = plan.execute_batch_raw([lengths[b] for b in sub], [d_in[b] for b in sub], 2, f_out, 32,
+                                dd.DEVICE_POINTERS)
+    assert rc == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert plan.execute_batch_raw([lengths[b] for b in sub], [d_in[b] for b in sub], 2, f_out, 32,
+                                      dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    out["fdd_bf16_8beams"] = {"value": 1024 * sum(lengths[b] - md for b in sub) / el, "unit": "DM*samples/s",
+                              "ms_per_batch": 1e3 * el, "nfft": ftune["nfft"]}
+    del f_out, plan, d_in
+    torch.cuda.empty_cache()
+    return out
+
+
 def stream_leg(torch, dd, rounds: int = 3) -> dict:
     """Streaming front end: 32 beams delivering 1 s chunks (15625 samples at
     64 us) of 1024 channels from 1530 MHz down in 0.3125 MHz steps, 8-bit
@@ -292,6 +370,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fdd", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fdd-forms", default="fp32,bf16,fp16",
                     help="contraction forms of the FDD leg (counter runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -390,6 +469,9 @@ def main():
     if not args.no_fdd:
         line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1 if args.warmup else 0,
                               forms=args.fdd_forms.split(","))
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        line["batch"] = batch_leg(torch, dd)
         torch.cuda.empty_cache()
     if not args.no_stream:
         line["stream"] = stream_leg(torch, dd)
