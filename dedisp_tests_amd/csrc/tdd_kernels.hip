@@ -56,7 +56,7 @@ __device__ __forceinline__ void tdd_store(uint8_t* row, size_t t, float scaled,
 // ---------------------------------------------------------------------------
 
 constexpr int TR_TS = 256;  // time samples per tile
-constexpr int TR_CB = 64;   // channel bytes per tile
+constexpr int TR_CB = 128;  // channel bytes per tile: whole 128-byte lines of a sample row
 constexpr int TR_PITCH = TR_CB + 4;
 
 template <int NBITS>
@@ -67,19 +67,39 @@ __global__ __launch_bounds__(256) void tdd_transpose_k(
     constexpr int SPW = 32 / NBITS;                // samples per output word
     constexpr int CH_PER_TILE = TR_CB * 8 / NBITS; // channels per tile
     constexpr int WPC = TR_TS / SPW;               // output words per channel per tile
-    __shared__ uint8_t tile[TR_TS][TR_PITCH];
+    __shared__ __attribute__((aligned(16))) uint8_t tile[TR_TS][TR_PITCH];
 
     const size_t t0 = (size_t)blockIdx.x * TR_TS;
     const size_t b0 = (size_t)blockIdx.y * TR_CB;
     const int tid = threadIdx.x;
 
-    // Load TR_TS rows x TR_CB bytes (zero outside the input).
-    for (int i = tid; i < TR_TS * TR_CB; i += 256) {
-        int tl = i / TR_CB, bl = i % TR_CB;
-        size_t t = t0 + tl, b = b0 + bl;
-        uint8_t v = 0;
-        if (t < nsamps && b < row_bytes) v = in[t * in_stride + b];
-        tile[tl][bl] = v;
+    // Load TR_TS rows x TR_CB bytes (zero outside the input): 32-bit words
+    // when the rows are word-aligned (32 lanes per 128-byte line), else bytes.
+    const bool words = ((in_stride | reinterpret_cast<uintptr_t>(in)) & 3u) == 0;
+    if (words) {
+        for (int i = tid; i < TR_TS * (TR_CB / 4); i += 256) {
+            const int tl = i / (TR_CB / 4), bl = 4 * (i % (TR_CB / 4));
+            const size_t t = t0 + tl, b = b0 + bl;
+            uint32_t v = 0;
+            if (t < nsamps) {
+                if (b + 3 < row_bytes) {
+                    v = *reinterpret_cast<const uint32_t*>(in + t * in_stride + b);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (b + q < row_bytes) v |= (uint32_t)in[t * in_stride + b + q] << (8 * q);
+                }
+            }
+            *reinterpret_cast<uint32_t*>(&tile[tl][bl]) = v;
+        }
+    } else {
+        for (int i = tid; i < TR_TS * TR_CB; i += 256) {
+            int tl = i / TR_CB, bl = i % TR_CB;
+            size_t t = t0 + tl, b = b0 + bl;
+            uint8_t v = 0;
+            if (t < nsamps && b < row_bytes) v = in[t * in_stride + b];
+            tile[tl][bl] = v;
+        }
     }
     __syncthreads();
 
