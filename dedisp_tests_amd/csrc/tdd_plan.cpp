@@ -77,16 +77,17 @@ dd_tdd_tiling default_tiling(unsigned nbits)
     // 29.5 ms with 8 waves (4 DMs/wave, 4 waves: 32.4 ms); 8-bit, 1024 ch x
     // 1000 DMs, 16 samples/lane: 44.4 ms (8 samples/lane, 4 DMs: 50.8 ms);
     // 2-bit, 512 ch x 1024 DMs: 12.8 ms (13.3 ms); 1-bit, 512 ch x 1024 DMs:
-    // 11.9 ms (13.7 ms).
+    // 11.9 ms (13.7 ms). 4-bit with 32-channel chunks: 28.8 ms (16: 29.9,
+    // 64: 42.5).
     t.dms_per_wave = 2;
     t.waves = 4;
+    t.chunk_channels = 16;
     switch (nbits) {
     case 1: t.samples_per_lane = 32; break;
     case 2: t.samples_per_lane = 16; break;
-    case 4: t.samples_per_lane = 16; t.waves = 8; break;
+    case 4: t.samples_per_lane = 16; t.waves = 8; t.chunk_channels = 32; break;
     default: t.samples_per_lane = 16; break;
     }
-    t.chunk_channels = 16;
     return t;
 }
 
@@ -572,8 +573,14 @@ double TDDPlan::autotune(size_type nsamps, size_type in_nbits, size_type out_nbi
     GPUContext& ctx = *m_ctx;
     TDDState& st = *m_state;
     ctx.make_current();
-    dd_tdd_tiling cand[64];
-    const int ncand = std::min(dd_tdd_lds_tilings(nbits, cand, 64), 64);
+    // every instantiated shape with 16- and 32-channel staging chunks
+    dd_tdd_tiling cand[128];
+    const int nshape = std::min(dd_tdd_lds_tilings(nbits, cand, 64), 64);
+    for (int i = 0; i < nshape; ++i) {
+        cand[nshape + i] = cand[i];
+        cand[nshape + i].chunk_channels = 32;
+    }
+    const int ncand = 2 * nshape;
     const size_t row_bytes = (m_nchans * nbits + 7) / 8;
     const size_t nsc = nsamps - m_max_delay, out_bytes = out_nbits / 8;
     DeviceBuffer d_in, d_out;
