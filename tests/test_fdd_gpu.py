@@ -158,6 +158,26 @@ def test_fdd_contraction_forms_vs_oracle(dd, orc, prec):
     assert abs(snr(out[i]) / snr(ref[i]) - 1) < 5e-3
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_fdd_forms_ragged_channels_many_chunks(dd, orc, prec):
+    # 100 channels (padded to 112 channel rows in the contraction), 2^13-sample
+    # segments over 4e5 samples: 51 chunks, i.e. a full and a partly filled
+    # group of 32 chunks; 150 DMs fill part of a DM block
+    p = {"fp32": dd.FDD_FP32, "bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec]
+    rng = np.random.default_rng(17)
+    nch, nsamps = 100, 400000
+    dt, f0, df = 64e-6, 1581.8, -1.0
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 150, 150, dtype=f32)
+    plan = _fdd_plan(dd, nch, dt, f0, df, nfft=8192, prec=p)
+    plan.set_dm_list(dms)
+    assert (nsamps - plan.max_delay + (8192 - plan.max_delay) - 1) // (8192 - plan.max_delay) > 32
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(out, ref) < PREC_TOL[prec]
+
+
 def test_fdd_valu_and_mfma_fp32_agree(dd, orc, monkeypatch):
     rng = np.random.default_rng(12)
     nch, nsamps = 96, 40000
