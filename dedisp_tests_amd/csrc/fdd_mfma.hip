@@ -63,7 +63,10 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 #endif
 
 constexpr int WAVES = DD_MFMA_WAVES;   // waves per workgroup, one 32-DM tile each
-constexpr int NB = 4;                  // bins per workgroup
+#ifndef DD_MFMA_NB
+#define DD_MFMA_NB 4
+#endif
+constexpr int NB = DD_MFMA_NB;         // bins per workgroup
 constexpr int KS = 16;                 // channels per K-step
 constexpr int NJ = 32;                 // chunks per tile
 constexpr int SLAB = NB * 2 * NJ * 8;  // elements of one plane of a K-step: [b][g][j][s]
@@ -132,6 +135,8 @@ template <int PREC>
 __global__ __launch_bounds__(NT)
 #if DD_MFMA_OCC2
 __attribute__((amdgpu_waves_per_eu(2, 2)))
+#elif DD_MFMA_OCC1
+__attribute__((amdgpu_waves_per_eu(1, 1)))
 #endif
 void fdd_mfma_k(
     const float2* __restrict__ X, size_t Xp, const uint16_t* __restrict__ q16, size_t qpitch,
@@ -171,14 +176,22 @@ void fdd_mfma_k(
     // read a valid element and are multiplied by 0), so the compiler has no
     // reason to wait for the loads before they are consumed at the end of the
     // K-step.
-    const int sb = tid & 3, sj = (tid >> 2) & 31, sg = (tid >> 7) & 1, sh0 = (tid >> 8) * HG;
+    // A thread stages bin sb of chunk sj for CPT consecutive channels sc0 ..
+    // (consecutive threads take consecutive bins: contiguous in X).
+    constexpr int CPT = 4 * HG;
+    static_assert(NT % (NB * NJ) == 0 && (NT / (NB * NJ)) * CPT == KS, "staging covers a K-step");
+    const int sb = tid % NB, sj = (tid / NB) % NJ, sc0 = (tid / (NB * NJ)) * CPT;
     const uint32_t sk = k0 + sb, sch = ch0 + sj;
     const float xs_scale = (sk <= M && sch < nchunk) ? xscale : 0.f;
-    const int s_off = ((sb * 2 + sg) * NJ + sj) * 8 + 4 * sh0;  // element offset in a plane
-    // element (channel 8g + 4h0 + q, chunk, bin) of the K-step: xsrc + q*rs,
-    // a K-step further 16*rs
+    // element offset in a plane of channels sc0 + 4u .. +3: [b][g][j][s], channel 8g + s
+    auto s_off = [&](int u) {
+        const int c = sc0 + 4 * u;
+        return ((sb * 2 + (c >> 3)) * NJ + sj) * 8 + (c & 7);
+    };
+    // element (channel sc0 + q, chunk, bin) of the K-step: xsrc + q*rs, a
+    // K-step further 16*rs
     const size_t rs = (size_t)nchunk * Xp;
-    const float2* xsrc = X + ((size_t)(8 * sg + 4 * sh0) * nchunk + min(sch, nchunk - 1)) * Xp + min(sk, M);
+    const float2* xsrc = X + ((size_t)sc0 * nchunk + min(sch, nchunk - 1)) * Xp + min(sk, M);
 
     float2 xv[4 * HG];
     auto load_x = [&]() {
@@ -194,7 +207,7 @@ void fdd_mfma_k(
             xr[q] = xv[4 * u + q].x * xs_scale;
             xi[q] = xv[4 * u + q].y * xs_scale;
         }
-        const int off = s_off + 4 * u;
+        const int off = s_off(u);
         if constexpr (PREC == 0) {
             float4* p = reinterpret_cast<float4*>(buf) + off / 4;
             p[0] = make_float4(xr[0], xr[1], xr[2], xr[3]);
