@@ -290,6 +290,8 @@ dedisp_error dedisp_amd_execute_batch(dedisp_plan plan, dedisp_size nbeams, cons
     if (nbeams && (!nsamps || !in || !out)) return DEDISP_INVALID_POINTER;
     if (dedisp::TDDPlan* t = as_tdd(plan))
         return guarded([&] { t->execute_batch(nbeams, nsamps, in, in_nbits, out, out_nbits, flags); });
+    if (dedisp::FDDGPUPlan* f = as_fdd(plan))
+        return guarded([&] { f->execute_batch(nbeams, nsamps, in, in_nbits, out, out_nbits, flags); });
     // other plans: one beam after another
     return guarded([&] {
         for (dedisp_size b = 0; b < nbeams; ++b)

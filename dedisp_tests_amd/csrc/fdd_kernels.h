@@ -12,6 +12,37 @@ extern "C" {
 // nfft values the on-chip transform engine handles.
 int dd_fdd_fft_supported(size_t nfft);
 
+// One overlap-save chunk of one beam, for transforms over the chunks of
+// several beams at once (all beams share nfft, the channels and the DM list;
+// the chunk's good length is the launch's). Forward: the beam's channel rows
+// (row_bytes per channel), the chunk's first sample t0, the beam's sample
+// count nvalid (zero beyond). Inverse: the beam's output (DM-major, out_stride
+// floats per DM row), the chunk's first output sample t0, the beam's outputs
+// per DM nsamps_out.
+typedef struct {
+    const uint8_t* rows;
+    float* out;
+    uint64_t row_bytes;
+    uint64_t out_stride;
+    uint32_t t0;
+    uint32_t nvalid;
+    uint32_t nsamps_out;
+    uint32_t reserved;
+} dd_fdd_chunk;
+
+// dd_fdd_forward over a chunk table: chunks [nchunk] on the device; X:
+// [nchan][nchunk][Xp], chunk i of X from chunks[i]. No periodic reads.
+hipError_t dd_fdd_forward_chunks(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
+                                 const float2* ph, size_t nfft, float2* X, size_t nchan, size_t Xp,
+                                 unsigned nbits, hipStream_t stream);
+
+// dd_fdd_inverse over a chunk table: Y row r (r < nrows), chunk i goes to DM
+// row row0 + r of chunks[i].out, samples chunks[i].t0 + s for s < good and
+// below chunks[i].nsamps_out.
+hipError_t dd_fdd_inverse_chunks(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
+                                 size_t nchunk, size_t Yp, const float2* ph, size_t nfft, size_t good,
+                                 hipStream_t stream);
+
 // Forward transforms of every (channel, chunk): rows are channel-major
 // nbits-bit samples (1, 2, 4 or 8; packed LSB-first along time in 32-bit
 // words, row_bytes per channel), chunk i starts at sample i*chunk_step and

@@ -37,6 +37,7 @@ struct FDDState {
     DeviceBuffer d_Z;       // [nchan padded to 16][nchunk][Xp] spectra, bins 0..nfft/2
     DeviceBuffer d_Y;       // [dm batch][nchunk][Yp] dedispersed spectra
     DeviceBuffer d_out;     // staged output (host-pointer mode)
+    DeviceBuffer d_chunks;  // [chunks of a batch] dd_fdd_chunk
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
     DeviceBuffer d_q16;     // [ndm][qpitch] uint16 phase steps d*65536/nfft mod 65536
     DeviceBuffer d_tab;     // [512] e^{2 pi i 256a/65536}, e^{2 pi i b/65536}
@@ -64,6 +65,8 @@ namespace {
 
 // Memory the dedispersed spectra of one DM batch may take.
 constexpr size_t kYBudget = size_t(24) << 30;
+// Memory the channel spectra of one group of beams may take (batched beams).
+constexpr size_t kZBudget = size_t(32) << 30;
 
 bool env_flag(const char* name)
 {
@@ -172,6 +175,123 @@ dedisp_size FDDGPUPlan::output_length(dedisp_size nsamps) const
     return nsamps - m_max_delay;
 }
 
+// Per-plan tables: truncated delays, channel scales (rebuilt when the DM list,
+// killmask or channels change).
+void FDDGPUPlan::prepare_tables()
+{
+    FDDState& st = *m_state;
+    hipStream_t stream = m_ctx->exec;
+    const size_t nchan = m_nchans, ndm = m_dm_count;
+    if (st.version != m_tables_version) {
+        std::vector<int32_t> d(ndm * nchan);
+        int32_t dmin = 0, dmax = 0;
+        for (size_t i = 0; i < ndm; ++i)
+            for (size_t c = 0; c < nchan; ++c) {
+                int32_t v = fdd_delay(h_dm_list[i], h_delay_table[c]);
+                d[i * nchan + c] = v;
+                dmin = std::min(dmin, v);
+                dmax = std::max(dmax, v);
+            }
+        // x = (u8 - 127.5) * (float)(1.0/nchan) (src/fdd/FDDGPUPlan.cpp:465-473);
+        // killed channels contribute nothing
+        std::vector<float> scale(nchan);
+        for (size_t c = 0; c < nchan; ++c) scale[c] = h_killmask[c] ? (float)(1.0 / nchan) : 0.f;
+        st.d_delays.reserve(d.size() * 4);
+        st.d_scale.reserve(nchan * 4);
+        DD_HIP(hipMemcpyAsync(st.d_delays.ptr, d.data(), d.size() * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_scale.ptr, scale.data(), nchan * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.dmin = dmin;
+        st.dmax = dmax;
+        st.h_delays.swap(d);
+        st.max_scale = nchan ? *std::max_element(scale.begin(), scale.end()) : 0.f;
+        st.version = m_tables_version;
+    }
+    if (st.dmin < 0) DD_THROW(DEDISP_UNKNOWN_ERROR);  // negative DMs: not supported
+}
+
+// Overlap-save segment length for series with nsc[0..n) outputs per DM: the
+// requested one, else the supported length with the smallest summed
+// contraction volume (chunks * (nfft/2 + 1)).
+size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
+{
+    const FDDState& st = *m_state;
+    const size_t dmax = (size_t)st.dmax;
+    size_t nfft = st.nfft_req;
+    if (!nfft) {
+        // smallest contraction volume nchunk * (nfft/2 + 1) among the supported lengths
+        size_t best = ~size_t(0);
+        for (size_t cand = 8192; cand <= 65536; cand *= 2) {
+            if (cand < dmax + 2 + 2) continue;
+            const size_t good = (cand - dmax) & ~size_t(1);
+            size_t chunks = 0;
+            for (size_t i = 0; i < n; ++i) chunks += div_up(nsc[i], good);
+            const size_t cost = chunks * (cand / 2 + 1);
+            if (cost < best) { best = cost; nfft = cand; }
+        }
+        if (!nfft) DD_THROW(DEDISP_UNKNOWN_ERROR);  // max delay beyond the largest segment
+    }
+    if (nfft < dmax + 4) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    return nfft;
+}
+
+// Phase tables of segment length nfft: e^{2 pi i m/nfft} for the transforms
+// and, for the matrix-core contraction (valu false), the 65536-based phase
+// steps and the two-level phasor tables. Returns the float16 spectrum scale.
+float FDDGPUPlan::prepare_phasors(size_t nfft, bool valu)
+{
+    FDDState& st = *m_state;
+    hipStream_t stream = m_ctx->exec;
+    const size_t nchan = m_nchans, ndm = m_dm_count;
+    if (st.ph_nfft != nfft) {
+        std::vector<float> ph(2 * nfft);
+        for (size_t m = 0; m < nfft; ++m) {
+            const double a = 2.0 * M_PI * (double)m / (double)nfft;
+            ph[2 * m] = (float)std::cos(a);
+            ph[2 * m + 1] = (float)std::sin(a);
+        }
+        st.d_ph.reserve(nfft * 8);
+        DD_HIP(hipMemcpyAsync(st.d_ph.ptr, ph.data(), nfft * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.ph_nfft = nfft;
+    }
+
+    // phase steps of the matrix-core contraction (65536-based phases)
+    if (!valu && (st.q16_version != m_tables_version || st.q16_nfft != nfft)) {
+        const size_t qpitch = round_up(nchan, 16);
+        const uint32_t mul = (uint32_t)(65536 / nfft);
+        std::vector<uint16_t> q(ndm * qpitch, 0);
+        for (size_t i = 0; i < ndm; ++i)
+            for (size_t c = 0; c < nchan; ++c)
+                q[i * qpitch + c] = (uint16_t)(((uint32_t)st.h_delays[i * nchan + c] * mul) & 0xffffu);
+        std::vector<float> tab(1024);
+        for (int a = 0; a < 256; ++a) {
+            const double t1 = 2.0 * M_PI * (double)(256 * a) / 65536.0;
+            const double t2 = 2.0 * M_PI * (double)a / 65536.0;
+            tab[2 * a] = (float)std::cos(t1);
+            tab[2 * a + 1] = (float)std::sin(t1);
+            tab[512 + 2 * a] = (float)std::cos(t2);
+            tab[512 + 2 * a + 1] = (float)std::sin(t2);
+        }
+        st.d_q16.reserve(q.size() * 2);
+        st.d_tab.reserve(tab.size() * 4);
+        DD_HIP(hipMemcpyAsync(st.d_q16.ptr, q.data(), q.size() * 2, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_tab.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        st.qpitch = qpitch;
+        st.q16_nfft = nfft;
+        st.q16_version = m_tables_version;
+    }
+    // float16 spectra: |X| <= nfft * 127.5 * max scale; a power of two brings
+    // that bound to at most 2^15 (float16 max 65504)
+    float xscale = 1.f;
+    if (st.precision == DEDISP_AMD_FDD_FP16 && st.max_scale > 0.f) {
+        const double bound = (double)nfft * 127.5 * (double)st.max_scale;
+        xscale = (float)std::ldexp(1.0, (int)std::floor(std::log2(32768.0 / bound)));
+    }
+    return xscale;
+}
+
 void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
                          byte_type* out, size_type out_nbits, unsigned flags)
 {
@@ -218,102 +338,17 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     if (nsc == 0) return;
     if (fourier && Nf >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
 
-    // ---- per-plan tables: truncated delays and channel scales ----
-    if (st.version != m_tables_version) {
-        std::vector<int32_t> d(ndm * nchan);
-        int32_t dmin = 0, dmax = 0;
-        for (size_t i = 0; i < ndm; ++i)
-            for (size_t c = 0; c < nchan; ++c) {
-                int32_t v = fdd_delay(h_dm_list[i], h_delay_table[c]);
-                d[i * nchan + c] = v;
-                dmin = std::min(dmin, v);
-                dmax = std::max(dmax, v);
-            }
-        // x = (u8 - 127.5) * (float)(1.0/nchan) (src/fdd/FDDGPUPlan.cpp:465-473);
-        // killed channels contribute nothing
-        std::vector<float> scale(nchan);
-        for (size_t c = 0; c < nchan; ++c) scale[c] = h_killmask[c] ? (float)(1.0 / nchan) : 0.f;
-        st.d_delays.reserve(d.size() * 4);
-        st.d_scale.reserve(nchan * 4);
-        DD_HIP(hipMemcpyAsync(st.d_delays.ptr, d.data(), d.size() * 4, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipMemcpyAsync(st.d_scale.ptr, scale.data(), nchan * 4, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipStreamSynchronize(stream));
-        st.dmin = dmin;
-        st.dmax = dmax;
-        st.h_delays.swap(d);
-        st.max_scale = nchan ? *std::max_element(scale.begin(), scale.end()) : 0.f;
-        st.version = m_tables_version;
-    }
-    if (st.dmin < 0) DD_THROW(DEDISP_UNKNOWN_ERROR);  // negative DMs: not supported
+    prepare_tables();
     const size_t dmax = (size_t)st.dmax;
-
-    // ---- chunking ----
-    size_t nfft = st.nfft_req;
-    if (!nfft) {
-        // smallest contraction volume nchunk * (nfft/2 + 1) among the supported lengths
-        size_t best = ~size_t(0);
-        for (size_t cand = 8192; cand <= 65536; cand *= 2) {
-            if (cand < dmax + 2 + 2) continue;
-            const size_t good = (cand - dmax) & ~size_t(1);
-            const size_t cost = div_up(nsc, good) * (cand / 2 + 1);
-            if (cost < best) { best = cost; nfft = cand; }
-        }
-        if (!nfft) DD_THROW(DEDISP_UNKNOWN_ERROR);  // max delay beyond the largest segment
-    }
-    if (nfft < dmax + 4) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    const size_t nfft = segment_length(&nsc, 1);
     const size_t good = (nfft - dmax) & ~size_t(1);
     const size_t nchunk = div_up(nsc, good);
     const size_t M = nfft / 2;
     const size_t Xp = M + 64, Yp = M + 64;  // bins 0..M, rows 512-byte aligned
 
-    if (st.ph_nfft != nfft) {
-        std::vector<float> ph(2 * nfft);
-        for (size_t m = 0; m < nfft; ++m) {
-            const double a = 2.0 * M_PI * (double)m / (double)nfft;
-            ph[2 * m] = (float)std::cos(a);
-            ph[2 * m + 1] = (float)std::sin(a);
-        }
-        st.d_ph.reserve(nfft * 8);
-        DD_HIP(hipMemcpyAsync(st.d_ph.ptr, ph.data(), nfft * 8, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipStreamSynchronize(stream));
-        st.ph_nfft = nfft;
-    }
-    const float2* ph = st.d_ph.as<float2>();
-
-    // phase steps of the matrix-core contraction (65536-based phases)
     const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
-    if (!valu && (st.q16_version != m_tables_version || st.q16_nfft != nfft)) {
-        const size_t qpitch = round_up(nchan, 16);
-        const uint32_t mul = (uint32_t)(65536 / nfft);
-        std::vector<uint16_t> q(ndm * qpitch, 0);
-        for (size_t i = 0; i < ndm; ++i)
-            for (size_t c = 0; c < nchan; ++c)
-                q[i * qpitch + c] = (uint16_t)(((uint32_t)st.h_delays[i * nchan + c] * mul) & 0xffffu);
-        std::vector<float> tab(1024);
-        for (int a = 0; a < 256; ++a) {
-            const double t1 = 2.0 * M_PI * (double)(256 * a) / 65536.0;
-            const double t2 = 2.0 * M_PI * (double)a / 65536.0;
-            tab[2 * a] = (float)std::cos(t1);
-            tab[2 * a + 1] = (float)std::sin(t1);
-            tab[512 + 2 * a] = (float)std::cos(t2);
-            tab[512 + 2 * a + 1] = (float)std::sin(t2);
-        }
-        st.d_q16.reserve(q.size() * 2);
-        st.d_tab.reserve(tab.size() * 4);
-        DD_HIP(hipMemcpyAsync(st.d_q16.ptr, q.data(), q.size() * 2, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipMemcpyAsync(st.d_tab.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipStreamSynchronize(stream));
-        st.qpitch = qpitch;
-        st.q16_nfft = nfft;
-        st.q16_version = m_tables_version;
-    }
-    // float16 spectra: |X| <= nfft * 127.5 * max scale; a power of two brings
-    // that bound to at most 2^15 (float16 max 65504)
-    float xscale = 1.f;
-    if (st.precision == DEDISP_AMD_FDD_FP16 && st.max_scale > 0.f) {
-        const double bound = (double)nfft * 127.5 * (double)st.max_scale;
-        xscale = (float)std::ldexp(1.0, (int)std::floor(std::log2(32768.0 / bound)));
-    }
+    const float xscale = prepare_phasors(nfft, valu);
+    const float2* ph = st.d_ph.as<float2>();
 
     // ---- input ----
     const uint8_t* d_in;
@@ -427,6 +462,198 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
         DD_HIP(hipStreamSynchronize(stream));
         ctx.timer.collect(ctx.profile);
         ctx.profile.work_units = (double)ndm * (double)nchan * (double)nchunk * (double)(M + 1);
+    }
+}
+
+// Batched beams, time-series output: every beam's overlap-save chunks (one
+// segment length for the batch) are concatenated along the chunk axis, so the
+// forward transforms, the contraction over the DM list and the inverse
+// transforms each run once per group of beams (a group's channel spectra
+// within kZBudget), with per-chunk tables giving each chunk its beam's rows
+// and output. A (DM, chunk) output sums the same products in the same order
+// as in a single-beam execute, so each beam's output equals execute() on it
+// alone with the same segment length.
+void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byte_type* const* in,
+                               size_type in_nbits, byte_type* const* out, size_type out_nbits,
+                               unsigned flags)
+{
+    if (nbeams == 0) return;
+    if (!nsamps || !in || !out) DD_THROW(DEDISP_INVALID_POINTER);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+        DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if ((m_nchans * in_nbits) % 8) DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    for (size_t b = 0; b < nbeams; ++b) {
+        if (!in[b] || !out[b]) DD_THROW(DEDISP_INVALID_POINTER);
+        if (nsamps[b] < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+        if (nsamps[b] >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    }
+    const bool segmented = m_state->segmented || env_flag("USE_SEGMENTED");
+    if (m_state->output == DEDISP_AMD_FDD_FOURIER && !segmented) {
+        // whole-series spectra: one beam after another
+        for (size_t b = 0; b < nbeams; ++b) execute_fdd(nsamps[b], in[b], in_nbits, out[b], flags, true);
+        return;
+    }
+
+    GPUContext& ctx = *m_ctx;
+    FDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    ctx.timer.reset(ctx.profiling);
+    ctx.profile = dedisp_amd_profile{};
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const size_t nchan = m_nchans, ndm = m_dm_count;
+    const size_t in_row = nchan * in_nbits / 8;
+
+    prepare_tables();
+    const size_t dmax = (size_t)st.dmax;
+    std::vector<size_t> nsc(nbeams);
+    for (size_t b = 0; b < nbeams; ++b) nsc[b] = nsamps[b] - m_max_delay;
+    const size_t nfft = segment_length(nsc.data(), nbeams);
+    const size_t good = (nfft - dmax) & ~size_t(1);
+    const size_t M = nfft / 2;
+    const size_t Xp = M + 64, Yp = M + 64;
+    const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
+    const float xscale = prepare_phasors(nfft, valu);
+    const float2* ph = st.d_ph.as<float2>();
+    const size_t nchan_pad = round_up(nchan, 16);
+
+    // ---- layout: groups of beams, each beam's rows / staged input / output ----
+    // DEDISP_FDD_GROUP_CHUNKS caps the chunks of a group below the memory bound
+    size_t max_chunks = std::max<size_t>(1, kZBudget / (nchan_pad * Xp * 8));
+    if (const char* e = std::getenv("DEDISP_FDD_GROUP_CHUNKS"))
+        if (std::atol(e) > 0) max_chunks = std::min(max_chunks, (size_t)std::atol(e));
+    struct Group {
+        size_t b0 = 0, b1 = 0, chunk0 = 0, nchunk = 0, rows = 0, in_bytes = 0, out_floats = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<size_t> nch(nbeams), rw(nbeams), roff(nbeams), ioff(nbeams), ooff(nbeams);
+    const size_t spw = 32 / in_nbits;  // samples per 32-bit row word
+    Group cur;
+    size_t total_chunks = 0;
+    double work = 0;
+    for (size_t b = 0; b < nbeams; ++b) {
+        nch[b] = div_up(nsc[b], good);
+        if (cur.b1 > cur.b0 && cur.nchunk + nch[b] > max_chunks) {
+            groups.push_back(cur);
+            cur = Group{};
+            cur.b0 = cur.b1 = b;
+            cur.chunk0 = total_chunks;
+        }
+        const size_t span = nch[b] ? std::max(nsamps[b], (nch[b] - 1) * good + nfft) : 0;
+        rw[b] = div_up(span, spw * 64) * 64;
+        roff[b] = cur.rows;
+        cur.rows += nchan * rw[b];
+        ioff[b] = cur.in_bytes;
+        cur.in_bytes += round_up(nsamps[b] * in_row, 256);
+        ooff[b] = cur.out_floats;
+        cur.out_floats += round_up(ndm * nsc[b], 64);
+        cur.nchunk += nch[b];
+        cur.b1 = b + 1;
+        total_chunks += nch[b];
+        work += (double)ndm * (double)nchan * (double)nch[b] * (double)(M + 1);
+    }
+    groups.push_back(cur);
+    if (total_chunks == 0) return;
+    size_t g_rows = 0, g_in = 0, g_out = 0, g_chunks = 0;
+    for (const Group& g : groups) {
+        g_rows = std::max(g_rows, g.rows);
+        g_in = std::max(g_in, g.in_bytes);
+        g_out = std::max(g_out, g.out_floats);
+        g_chunks = std::max(g_chunks, g.nchunk);
+    }
+    const size_t per_dm = g_chunks * Yp * 8;
+    const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    st.d_rows.reserve(g_rows * 4);
+    st.d_Z.reserve(nchan_pad * g_chunks * Xp * 8);
+    st.d_Y.reserve(dm_batch * g_chunks * Yp * 8);
+    st.d_chunks.reserve(total_chunks * sizeof(dd_fdd_chunk));
+    if (!device_ptrs) {
+        st.d_in.reserve(g_in + 64);
+        st.d_out.reserve(g_out * 4);
+    }
+
+    // ---- chunk tables (one upload for the batch) ----
+    std::vector<dd_fdd_chunk> tab(total_chunks);
+    for (const Group& g : groups) {
+        size_t i = g.chunk0;
+        for (size_t b = g.b0; b < g.b1; ++b)
+            for (size_t j = 0; j < nch[b]; ++j, ++i) {
+                dd_fdd_chunk& e = tab[i];
+                e.rows = st.d_rows.as<uint8_t>() + roff[b] * 4;
+                e.out = device_ptrs ? reinterpret_cast<float*>(out[b]) : st.d_out.as<float>() + ooff[b];
+                e.row_bytes = rw[b] * 4;
+                e.out_stride = nsc[b];
+                e.t0 = (uint32_t)(j * good);
+                e.nvalid = (uint32_t)nsamps[b];
+                e.nsamps_out = (uint32_t)nsc[b];
+                e.reserved = 0;
+            }
+    }
+    DD_HIP(hipMemcpyAsync(st.d_chunks.ptr, tab.data(), tab.size() * sizeof(dd_fdd_chunk),
+                          hipMemcpyHostToDevice, stream));
+    DD_HIP(hipStreamSynchronize(stream));
+    const dd_fdd_chunk* d_tab = st.d_chunks.as<dd_fdd_chunk>();
+
+    for (const Group& g : groups) {
+        if (g.nchunk == 0) continue;
+        const size_t nc = g.nchunk;
+        if (!device_ptrs) {
+            ctx.timer.begin(StageTimer::H2D, stream);
+            for (size_t b = g.b0; b < g.b1; ++b)
+                if (nch[b])
+                    DD_HIP(hipMemcpyAsync(st.d_in.as<uint8_t>() + ioff[b], in[b], nsamps[b] * in_row,
+                                          hipMemcpyHostToDevice, stream));
+            ctx.timer.end(StageTimer::H2D, stream);
+        }
+        if (nchan_pad > nchan)
+            DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nc * Xp, 0, (nchan_pad - nchan) * nc * Xp * 8,
+                                  stream));
+        ctx.timer.begin(StageTimer::UNPACK, stream);
+        for (size_t b = g.b0; b < g.b1; ++b) {
+            if (!nch[b]) continue;
+            const uint8_t* src = device_ptrs ? in[b] : st.d_in.as<uint8_t>() + ioff[b];
+            DD_HIP(dd_tdd_transpose(src, nsamps[b], in_row, (unsigned)in_nbits, nchan,
+                                    st.d_rows.as<uint32_t>() + roff[b], rw[b], stream));
+        }
+        ctx.timer.end(StageTimer::UNPACK, stream);
+        ctx.timer.begin(StageTimer::FFT, stream);
+        DD_HIP(dd_fdd_forward_chunks(d_tab + g.chunk0, nc, st.d_scale.as<float>(), ph, nfft,
+                                     st.d_Z.as<float2>(), nchan, Xp, (unsigned)in_nbits, stream));
+        ctx.timer.end(StageTimer::FFT, stream);
+        for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
+            const size_t nb = std::min(dm_batch, ndm - b0);
+            ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+            if (valu)
+                DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, st.d_delays.as<int32_t>() + b0 * nchan, nb,
+                                            nchan, nc, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
+            else
+                DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Xp,
+                                            st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan, nc,
+                                            st.d_tab.as<float2>(), nfft, xscale, st.d_Y.as<float2>(), Yp,
+                                            stream));
+            ctx.timer.end(StageTimer::DEDISPERSE, stream);
+            ctx.timer.begin(StageTimer::IFFT, stream);
+            DD_HIP(dd_fdd_inverse_chunks(st.d_Y.as<float2>(), nb, b0, d_tab + g.chunk0, nc, Yp, ph, nfft, good,
+                                         stream));
+            ctx.timer.end(StageTimer::IFFT, stream);
+        }
+        if (!device_ptrs) {
+            ctx.timer.begin(StageTimer::D2H, stream);
+            for (size_t b = g.b0; b < g.b1; ++b)
+                if (nch[b])
+                    DD_HIP(hipMemcpyAsync(out[b], st.d_out.as<float>() + ooff[b], ndm * nsc[b] * 4,
+                                          hipMemcpyDeviceToHost, stream));
+            ctx.timer.end(StageTimer::D2H, stream);
+        }
+    }
+    if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
+        DD_HIP(hipStreamSynchronize(stream));
+        ctx.timer.collect(ctx.profile);
+        ctx.profile.work_units = work;
     }
 }
 

@@ -217,11 +217,14 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
 // k = t + T3*k0 has its mirror M-k at element A-1-k0 of thread T3-t (t > 0; for
 // t = 0 at element (A-k0) mod A of thread 0). Two rounds per component, each
 // moving half the elements, keep the extra registers to A/2 per component.
-template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
+// TAB: the chunk's rows, first sample and valid length come from tab[chunk]
+// (the chunks of several beams in one launch) instead of rows/chunk_step/nvalid.
+template <int A, int B, int C, bool TAB = false, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_forward_k(
     const uint8_t* __restrict__ rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
     size_t period, const float* __restrict__ chan_scale, const float2* __restrict__ ph,
-    float2* __restrict__ X, size_t nchunk, size_t Xp, uint32_t nbits)
+    float2* __restrict__ X, size_t nchunk, size_t Xp, uint32_t nbits,
+    const dd_fdd_chunk* __restrict__ tab)
 {
     using S = FftShape<A, B, C>;
     static_assert(S::T3 == NT, "pass 3 uses every thread");
@@ -229,8 +232,15 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     static_assert(S::LDS_FLOATS >= M, "a pairing round fits the LDS");
     __shared__ float lds[S::LDS_FLOATS];
     const size_t chunk = blockIdx.x, c = blockIdx.y;
+    size_t t0 = chunk * chunk_step;
+    if constexpr (TAB) {
+        const dd_fdd_chunk e = tab[chunk];
+        rows = e.rows;
+        row_bytes = e.row_bytes;
+        nvalid = e.nvalid;
+        t0 = e.t0;
+    }
     const uint8_t* src = rows + c * row_bytes;
-    const size_t t0 = chunk * chunk_step;
     const float s = chan_scale[c];
     float2* dst = X + (c * nchunk + chunk) * Xp;
     // x[t] = (u8 - 127.5) * scale (src/fdd/unpack/unpack_kernel.cuh:60) for
@@ -312,10 +322,13 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
 // from the other half of the wave (v_permlane32_swap). Index T1/2 is its own
 // mirror partner (element C-1-n2 of itself); index 0 pairs element n2 with
 // its own element C-n2 (k = 0 with Y[M]), read back from a small LDS area.
-template <int A, int B, int C, int NT = FftShape<A, B, C>::T>
+// TAB: the chunk's output row (row row0 + blockIdx.y of tab[chunk].out), first
+// output sample and output length come from tab[chunk].
+template <int A, int B, int C, bool TAB = false, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_inverse_k(
     const float2* __restrict__ Y, size_t nchunk, size_t Yp, const float2* __restrict__ ph,
-    float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out)
+    float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out,
+    const dd_fdd_chunk* __restrict__ tab, size_t row0)
 {
     using S = FftShape<A, B, C>;
     constexpr int M = S::M, T1 = S::T1;
@@ -324,8 +337,15 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     __shared__ float lds[S::LDS_FLOATS];
     const size_t chunk = blockIdx.x, row = blockIdx.y;
     const float2* src = Y + (row * nchunk + chunk) * Yp;
-    float* dst = out + row * out_stride;
-    const size_t t0 = chunk * good;
+    size_t t0 = chunk * good;
+    if constexpr (TAB) {
+        const dd_fdd_chunk e = tab[chunk];
+        out = e.out;
+        out_stride = e.out_stride;
+        nsamps_out = e.nsamps_out;
+        t0 = e.t0;
+    }
+    float* dst = out + (row0 + row) * out_stride;
     const float inv_n = 1.0f / (float)(2 * M);
     const float ymx = src[M].x;  // Y[M]: its imaginary part is not used
     const int t = threadIdx.x;
@@ -403,7 +423,23 @@ hipError_t launch_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_st
     if (nbits != 8 && (row_bytes % 4)) return hipErrorInvalidValue;
     hipLaunchKernelGGL((fdd_forward_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nchan),
                        dim3(FftShape<A, B, C>::T), 0, stream, rows, row_bytes, chunk_step, nvalid,
-                       period, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits);
+                       period, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits,
+                       (const dd_fdd_chunk*)nullptr);
+    return hipGetLastError();
+}
+
+template <int A, int B, int C>
+hipError_t launch_forward_chunks(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
+                                 const float2* ph, float2* X, size_t nchan, size_t Xp, unsigned nbits,
+                                 hipStream_t stream)
+{
+    if (Xp < (size_t)FftShape<A, B, C>::M + 1 || !chunks) return hipErrorInvalidValue;
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return hipErrorInvalidValue;
+    if (!nchunk || !nchan) return hipSuccess;
+    hipLaunchKernelGGL((fdd_forward_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nchan),
+                       dim3(FftShape<A, B, C>::T), 0, stream, (const uint8_t*)nullptr, (size_t)0,
+                       (size_t)0, (size_t)0, (size_t)0, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits,
+                       chunks);
     return hipGetLastError();
 }
 
@@ -414,7 +450,20 @@ hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
 {
     hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nrows),
                        dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
-                       good, nsamps_out);
+                       good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
+    return hipGetLastError();
+}
+
+template <int A, int B, int C>
+hipError_t launch_inverse_chunks(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
+                                 size_t nchunk, size_t Yp, const float2* ph, size_t good,
+                                 hipStream_t stream)
+{
+    if (!chunks) return hipErrorInvalidValue;
+    if (!nchunk || !nrows) return hipSuccess;
+    hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nrows),
+                       dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
+                       (size_t)0, good, (size_t)0, chunks, row0);
     return hipGetLastError();
 }
 
@@ -530,6 +579,38 @@ extern "C" hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size
     case 32768: return DD_FWD(16, 32, 32);
     case 65536: return DD_FWD(32, 32, 32);
 #undef DD_FWD
+    default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t dd_fdd_forward_chunks(const dd_fdd_chunk* chunks, size_t nchunk,
+                                            const float* chan_scale, const float2* ph, size_t nfft,
+                                            float2* X, size_t nchan, size_t Xp, unsigned nbits,
+                                            hipStream_t stream)
+{
+#define DD_FWD(A, B, C) launch_forward_chunks<A, B, C>(chunks, nchunk, chan_scale, ph, X, nchan, Xp, nbits, stream)
+    switch (nfft) {
+    case 8192:  return DD_FWD(16, 16, 16);
+    case 16384: return DD_FWD(16, 16, 32);
+    case 32768: return DD_FWD(16, 32, 32);
+    case 65536: return DD_FWD(32, 32, 32);
+#undef DD_FWD
+    default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t dd_fdd_inverse_chunks(const float2* Y, size_t nrows, size_t row0,
+                                            const dd_fdd_chunk* chunks, size_t nchunk, size_t Yp,
+                                            const float2* ph, size_t nfft, size_t good,
+                                            hipStream_t stream)
+{
+#define DD_INV(A, B, C) launch_inverse_chunks<A, B, C>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream)
+    switch (nfft) {
+    case 8192:  return DD_INV(16, 16, 16);
+    case 16384: return DD_INV(16, 16, 32);
+    case 32768: return DD_INV(16, 32, 32);
+    case 65536: return DD_INV(32, 32, 32);
+#undef DD_INV
     default: return hipErrorInvalidValue;
     }
 }

@@ -208,9 +208,23 @@ public:
     double autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
                     dedisp_amd_tuning* chosen);
 
+    /* Batched execution (extension, dedisp_amd_execute_batch): as
+     * TDDPlan::execute_batch. In time-series mode the overlap-save chunks of
+     * all beams go through one forward transform, one contraction and one
+     * inverse transform per group of beams (one segment length for the
+     * batch), so short beams share the contraction's 32-chunk tiles; each
+     * beam's output equals execute() on that beam alone with the same segment
+     * length. Whole-series Fourier output runs beam by beam. */
+    void execute_batch(size_type nbeams, const size_type* nsamps, const byte_type* const* in,
+                       size_type in_nbits, byte_type* const* out, size_type out_nbits,
+                       unsigned flags = 0);
+
 private:
     void execute_fdd(size_type nsamps, const byte_type* in, size_type in_nbits, byte_type* out,
                      unsigned flags, bool fourier);
+    void prepare_tables();
+    size_t segment_length(const size_t* nsc, size_t n) const;
+    float prepare_phasors(size_t nfft, bool valu);
 
     std::unique_ptr<FDDState> m_state;
 };

@@ -180,7 +180,13 @@ dedisp_error dedisp_amd_write_inf(const char* prefix, const dedisp_amd_fil_heade
  * in_nbits <= 8 transpose each beam and run one channel-sum launch over all
  * beams; with DEDISP_DEVICE_POINTERS that sequence is captured in a HIP graph
  * and replayed while lengths, pointers, widths and DM list stay the same.
- * Other cases (and FDD plans) run the beams one after another. */
+ * FDD plans writing time series use one segment length for the batch (the
+ * set one, else the one with the smallest summed contraction volume) and run
+ * the overlap-save chunks of all beams through one forward transform, one
+ * contraction and one inverse transform per group of beams; each beam's
+ * output equals dedisp_execute on it alone with that segment length. Other
+ * cases (weighted killmasks, 16/32-bit TDD input, whole-series FDD spectra)
+ * run the beams one after another. */
 dedisp_error dedisp_amd_execute_batch(dedisp_plan plan, dedisp_size nbeams,
                                       const dedisp_size* nsamps, const dedisp_byte* const* in,
                                       dedisp_size in_nbits, dedisp_byte* const* out,

@@ -113,14 +113,86 @@ def test_batch_errors(dd):
     assert plan.execute_batch_raw([], [], 8, [], 32) == 0
 
 
+def _u8_beams(rng, lengths, nch):
+    return [np.clip(np.rint(rng.normal(128, 20, size=(n, nch))), 0, 255).astype(np.uint8) for n in lengths]
+
+
 def test_fdd_batch_matches_single(dd):
+    """Chunks of all beams in one forward / contraction / inverse sequence:
+    every beam equals its single execute at the same segment length."""
     nch = 64
     plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.FDD)
     plan.fdd_set_output(dd.FDD_TIMESERIES)
     plan.set_dm_list(np.linspace(0, 60, 9, dtype=f32))
+    plan.fdd_set_segmented(True, 8192)
     rng = np.random.default_rng(4)
-    beams = [np.clip(np.rint(rng.normal(128, 20, size=(n, nch))), 0, 255).astype(np.uint8)
-             for n in (9000, 14000, 6000)]
+    md = plan.max_delay
+    # uneven lengths; one beam without outputs (nsamps == max_delay)
+    beams = _u8_beams(rng, (9000, 14000, md, 6000), nch)
+    outs = plan.execute_batch(beams, 8, 32)
+    for i, b in enumerate(beams):
+        assert outs[i].shape == (plan.dm_count, b.shape[0] - md)
+        if b.shape[0] > md:
+            assert np.array_equal(outs[i], plan.execute(b, 8, 32)), i
+
+
+@pytest.mark.parametrize("prec", ["FDD_FP32", "FDD_BF16", "FDD_FP16"])
+def test_fdd_batch_groups_device_pointers(dd, prec, monkeypatch):
+    """Several groups of beams (the chunks per group capped at 5, so groups
+    split between beams and a beam longer than the cap forms its own group),
+    100 channels (padded to 112 in the contraction), every contraction form,
+    device pointers: each beam equals its single execute bit for bit."""
+    import torch
+    nch = 100
+    plan = dd.Plan(nch, 64e-6, 1581.8, -1.5, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    plan.fdd_set_precision(getattr(dd, prec))
+    plan.set_dm_list(np.linspace(0, 90, 70, dtype=f32))
+    plan.fdd_set_segmented(True, 8192)
+    md, ndm = plan.max_delay, plan.dm_count
+    rng = np.random.default_rng(12)
+    lengths = [20000, 9000, 61000, 15000, 30000, 12001]
+    host = _u8_beams(rng, lengths, nch)
+    d_in = [torch.from_numpy(h).cuda() for h in host]
+    d_out = [torch.empty((ndm, n - md), dtype=torch.float32, device="cuda") for n in lengths]
+    monkeypatch.setenv("DEDISP_FDD_GROUP_CHUNKS", "5")
+    torch.cuda.synchronize()
+    assert plan.execute_batch_raw(lengths, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DEDISP_FDD_GROUP_CHUNKS")
+    for i, n in enumerate(lengths):
+        one = torch.empty_like(d_out[i])
+        assert plan.execute_raw(n, d_in[i], 8, one, 32, dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(d_out[i], one), i
+
+
+def test_fdd_batch_subbyte_auto_segment(dd, orc):
+    """4-bit input, host pointers, segment length chosen for the batch: each
+    beam within the FDD-FP32 tolerance of the oracle."""
+    nch, dt, f0, df = 64, 64e-6, 1581.8, -2.0
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    dms = np.linspace(0, 60, 9, dtype=f32)
+    plan.set_dm_list(dms)
+    rng = np.random.default_rng(21)
+    lengths = [12000, 30001, 7000]
+    raw = [rng.integers(0, 16, size=(n, nch), dtype=np.uint32) for n in lengths]
+    beams = [dd.pack_samples(r, 4) for r in raw]
+    outs = plan.execute_batch(beams, 4, 32)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    for i, (r, n) in enumerate(zip(raw, lengths)):
+        ref = orc.fdd_timeseries(nch, delays, dms, n, plan.max_delay, r.astype(np.uint8), nbits=4)
+        err = np.linalg.norm(outs[i] - ref) / np.linalg.norm(ref)
+        assert err < 1e-4, (i, err)
+
+
+def test_fdd_batch_fourier_output_runs_beam_by_beam(dd):
+    nch = 64
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.FDD)
+    plan.set_dm_list(np.linspace(0, 30, 5, dtype=f32))
+    rng = np.random.default_rng(9)
+    beams = _u8_beams(rng, (5000, 7000), nch)
     outs = plan.execute_batch(beams, 8, 32)
     for i, b in enumerate(beams):
         assert np.array_equal(outs[i], plan.execute(b, 8, 32)), i
