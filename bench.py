@@ -20,8 +20,12 @@ Prints one JSON line. Besides the contract fields it carries:
   valu_roofline  the integer-add view of the same kernel (channel-adds / s
                  against the int32 VALU lane rate; the kernel packs 4 sums per
                  32-bit lane, so the fraction can exceed 1)
+  batch          64 beams of uneven length through one TDD graph replay, and
+                 FDD-BF16 on 8 of them with their chunks concatenated
   stream         the streaming front end on 32 beams x 1 s chunks, 1024 ch,
-                 3000 DMs: push latency percentiles and real-time factor
+                 3000 DMs: push latency percentiles and real-time factor, per
+                 beam (TDD) and with one batched push per round (TDD,
+                 FDD-BF16, FDD-FP32)
   cpu_baseline   the reference's CPU FDD plan (restated as written, OpenMP)
                  timed on this host on a bounded DM sample, scaled to the
                  full DM list
@@ -319,7 +323,8 @@ def stream_leg(torch, dd, rounds: int = 4) -> dict:
     64 us) of 1024 channels from 1530 MHz down in 0.3125 MHz steps, 8-bit
     N(128, 20), 3000 linear DM trials 0..1500, TDD plan, float
     output. Round 0 fills the histories and is not timed; every later round
-    pushes one chunk per beam with input and output in HBM."""
+    pushes one chunk per beam with input and output in HBM. Then the same
+    workload with one batched push per round: TDD, FDD-BF16 and FDD-FP32."""
     nb, chunk, nch, dt = 32, 15625, 1024, 64e-6
     plan = dd.Plan(nch, dt, 1530.0, -0.3125, impl=dd.TDD)
     plan.set_dm_list(np.linspace(0.0, 1500.0, 3000, dtype=np.float32))
@@ -348,18 +353,71 @@ def stream_leg(torch, dd, rounds: int = 4) -> dict:
     st.close()
     t_round = float(np.median(round_s))
     lat = np.sort(np.array(push_ms))
-    nearest = lambda q: float(lat[min(len(lat) - 1, max(0, int(np.ceil(q * len(lat))) - 1))])
-    return {
+    out = {
         "workload": "stream-32beam-1024ch-15625x8bit-3000dm", "beams": nb, "chunk_nsamps": chunk,
         "max_delay": plan.max_delay, "ndm": plan.dm_count, "timed_rounds": len(round_s),
         "round_ms": 1e3 * t_round,
         "real_time_factor": nb * chunk * dt / t_round,
         "dm_samples_per_s": plan.dm_count * chunk * nb / t_round,
-        "push_ms": {"p50": nearest(0.5), "p99": nearest(0.99), "mean": float(lat.mean()), "max": float(lat[-1]),
-                    "pushes": len(lat), "note": "timed rounds, host-synchronous pushes, nearest rank"},
+        "push_ms": {"p50": _nearest(lat, 0.5), "p99": _nearest(lat, 0.99), "mean": float(lat.mean()),
+                    "max": float(lat[-1]), "pushes": len(lat),
+                    "note": "timed rounds, host-synchronous pushes, nearest rank"},
         "library_stats_all_rounds": {"p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"],
                                      "pushes": stats["pushes"]},
     }
+    del d_out, st
+    torch.cuda.empty_cache()
+    out["tdd_batched"] = stream_batched(torch, dd, plan, rounds=6)
+    del plan
+    torch.cuda.empty_cache()
+    for form, prec in (("fdd_bf16_batched", dd.FDD_BF16), ("fdd_fp32_batched", dd.FDD_FP32)):
+        fplan = dd.Plan(nch, dt, 1530.0, -0.3125, impl=dd.FDD)
+        fplan.fdd_set_precision(prec)
+        fplan.set_dm_list(np.linspace(0.0, 1500.0, 3000, dtype=np.float32))
+        out[form] = stream_batched(torch, dd, fplan, rounds=4)
+        del fplan
+        torch.cuda.empty_cache()
+    return out
+
+
+def _nearest(sorted_vals, q: float) -> float:
+    """Nearest-rank percentile of a sorted array."""
+    n = len(sorted_vals)
+    return float(sorted_vals[min(n - 1, max(0, int(np.ceil(q * n)) - 1))])
+
+
+def stream_batched(torch, dd, plan, rounds: int) -> dict:
+    """The streaming workload with one batched push per round: the 32 beams'
+    1 s chunks of a round go through one dedisp_amd_stream_push_batch (one
+    channel-sum launch for TDD; for FDD the overlap-save chunks of all beams in
+    one forward transform / contraction / inverse sequence). Round 0 fills the
+    histories and is not timed; a chunk's latency is its round's call time."""
+    nb, chunk, nch, dt = 32, 15625, 1024, 64e-6
+    st = dd.Stream(plan, nb, chunk, 8, 32)
+    d_out = [torch.empty((plan.dm_count, chunk), dtype=torch.float32, device="cuda") for _ in range(nb)]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(6)
+    call_ms = []
+    for r in range(rounds):
+        x = torch.randn((nb, chunk, nch), generator=g, device="cuda", dtype=torch.float32)
+        chunks = torch.clamp(torch.round(x * 20.0 + 128.0), 0, 255).to(torch.uint8)
+        del x
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = st.push_batch_raw(list(range(nb)), [chunks[b] for b in range(nb)], d_out, dd.DEVICE_POINTERS)
+        if rc != 0:
+            raise dd.DedispError(rc, "dedisp_amd_stream_push_batch")
+        if r > 0:
+            call_ms.append(1e3 * (time.perf_counter() - t0))
+        del chunks
+    st.close()
+    lat = np.sort(np.array(call_ms))
+    t_round = float(np.median(lat)) * 1e-3
+    return {"timed_rounds": len(lat), "round_ms": 1e3 * t_round,
+            "real_time_factor": nb * chunk * dt / t_round,
+            "dm_samples_per_s": plan.dm_count * chunk * nb / t_round,
+            "chunk_latency_ms": {"p50": _nearest(lat, 0.5), "p99": _nearest(lat, 0.99), "max": float(lat[-1]),
+                                 "note": "one batched push per round; every chunk of a round completes with it"}}
 
 
 def valu_roofline(k_ms: float, adds_t: float) -> dict:

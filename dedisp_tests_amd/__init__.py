@@ -154,6 +154,8 @@ _SIG = [
     ("dedisp_amd_stream_create", C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, C.c_ulong, C.c_ulong,
                                            C.c_ulong, C.c_ulong]),
     ("dedisp_amd_stream_push", C.c_int, [C.c_void_p, C.c_ulong, C.c_void_p, C.c_void_p, C.c_uint]),
+    ("dedisp_amd_stream_push_batch", C.c_int, [C.c_void_p, C.c_ulong, C.POINTER(C.c_ulong),
+                                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint]),
     ("dedisp_amd_stream_reset", C.c_int, [C.c_void_p]),
     ("dedisp_amd_stream_get_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
     ("dedisp_amd_stream_history", C.c_ulong, [C.c_void_p]),
@@ -433,6 +435,22 @@ class Stream:
         out = np.empty((self.plan.dm_count, self.chunk), dtype=dt)
         _check(self.push_raw(beam, chunk, out), "dedisp_amd_stream_push")
         return out
+
+    def push_batch_raw(self, beams, chunk_ptrs, out_ptrs, flags: int = 0) -> int:
+        n = len(beams)
+        b = (C.c_ulong * n)(*beams)
+        ins = (C.c_void_p * n)(*[_ptr(x) for x in chunk_ptrs])
+        outs = (C.c_void_p * n)(*[_ptr(x) for x in out_ptrs])
+        return lib().dedisp_amd_stream_push_batch(self._h, n, b, ins, outs, flags)
+
+    def push_batch(self, beams, chunks) -> list:
+        """Host arrays: one chunk per beam (distinct beams) in one batched push;
+        returns one [ndm, chunk_nsamps] array per beam."""
+        chunks = [np.ascontiguousarray(c) for c in chunks]
+        dt = {8: np.uint8, 16: np.uint16, 32: np.float32}[self.out_nbits]
+        outs = [np.empty((self.plan.dm_count, self.chunk), dtype=dt) for _ in chunks]
+        _check(self.push_batch_raw(list(beams), chunks, outs), "dedisp_amd_stream_push_batch")
+        return outs
 
     def reset(self) -> None:
         _check(lib().dedisp_amd_stream_reset(self._h), "dedisp_amd_stream_reset")

@@ -7,7 +7,10 @@
 // runs the plan on the whole buffer with device pointers (chunk output
 // samples per DM), and copies the last max_delay samples to the head of the
 // other buffer, which becomes the beam's buffer for the next push. Nothing is
-// recomputed: the history is only re-read by the channel sum.
+// recomputed: the history is only re-read by the channel sum. A batched push
+// takes one chunk of each of several beams through the plan's batched
+// execution (one channel-sum launch, or one FDD transform / contraction
+// sequence over the chunks of all of them).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -29,6 +32,7 @@ struct dedisp_amd_stream_struct {
     dedisp::DeviceBuffer d_out;              // output staging for host outputs
     std::vector<double> lat_ms;
     double busy_s = 0;
+    double samples = 0;  // input samples pushed (all beams)
 };
 
 namespace {
@@ -151,6 +155,67 @@ dedisp_error dedisp_amd_stream_push(dedisp_amd_stream s, dedisp_size beam, const
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         s->lat_ms.push_back(dt * 1e3);
         s->busy_s += dt;
+        s->samples += (double)s->chunk;
+    });
+}
+
+dedisp_error dedisp_amd_stream_push_batch(dedisp_amd_stream s, dedisp_size n, const dedisp_size* beams,
+                                          const dedisp_byte* const* chunks, dedisp_byte* const* outs,
+                                          unsigned flags)
+{
+    if (!s) return DEDISP_INVALID_PLAN;
+    if (n == 0) return DEDISP_NO_ERROR;
+    if (!beams || !chunks || !outs) return DEDISP_INVALID_POINTER;
+    std::vector<char> seen(s->nbeams, 0);
+    for (size_t i = 0; i < n; ++i) {
+        if (!chunks[i] || !outs[i] || beams[i] >= s->nbeams || seen[beams[i]]) return DEDISP_INVALID_POINTER;
+        seen[beams[i]] = 1;
+    }
+    return guarded([&] {
+        if (s->plan->get_max_delay() != s->hist || s->plan->get_dm_count() != s->ndm)
+            DD_THROW(DEDISP_INVALID_PLAN);
+        const auto t0 = std::chrono::steady_clock::now();
+        dedisp::GPUContext& ctx = s->plan->context();
+        ctx.make_current();
+        const bool dev = (flags & DEDISP_DEVICE_POINTERS) != 0;
+        const size_t hist_bytes = s->hist * s->row_bytes, chunk_bytes = s->chunk * s->row_bytes;
+        const size_t out_bytes = s->ndm * s->chunk * (s->out_nbits / 8);
+        std::vector<dedisp_size> nsamps(n, s->hist + s->chunk);
+        std::vector<const dedisp_byte*> in(n);
+        std::vector<dedisp_byte*> dst(n);
+        if (!dev) s->d_out.reserve(n * out_bytes);
+        for (size_t i = 0; i < n; ++i) {
+            const size_t beam = beams[i];
+            dedisp::DeviceBuffer& a = s->bufs[2 * beam + s->cur[beam]];
+            dedisp::DeviceBuffer& b = s->bufs[2 * beam + 1 - s->cur[beam]];
+            auto* buf = a.as<dedisp_byte>();
+            DD_HIP(hipMemcpyAsync(buf + hist_bytes, chunks[i], chunk_bytes,
+                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx.exec));
+            if (s->hist)
+                DD_HIP(hipMemcpyAsync(b.ptr, buf + chunk_bytes, hist_bytes, hipMemcpyDeviceToDevice, ctx.exec));
+            in[i] = buf;
+            dst[i] = dev ? outs[i] : s->d_out.as<dedisp_byte>() + i * out_bytes;
+        }
+        // one batched execution: a channel-sum launch over every beam's tiles
+        // (TDD) or one chunk-concatenated transform/contraction sequence (FDD)
+        if (auto* t = dynamic_cast<dedisp::TDDPlan*>(s->plan))
+            t->execute_batch(n, nsamps.data(), in.data(), s->in_nbits, dst.data(), s->out_nbits,
+                             DEDISP_DEVICE_POINTERS);
+        else if (auto* f = dynamic_cast<dedisp::FDDGPUPlan*>(s->plan))
+            f->execute_batch(n, nsamps.data(), in.data(), s->in_nbits, dst.data(), s->out_nbits,
+                             DEDISP_DEVICE_POINTERS);
+        else
+            for (size_t i = 0; i < n; ++i)
+                s->plan->execute(nsamps[i], in[i], s->in_nbits, dst[i], s->out_nbits, DEDISP_DEVICE_POINTERS);
+        if (!dev)
+            for (size_t i = 0; i < n; ++i)
+                DD_HIP(hipMemcpyAsync(outs[i], dst[i], out_bytes, hipMemcpyDeviceToHost, ctx.exec));
+        DD_HIP(hipStreamSynchronize(ctx.exec));
+        for (size_t i = 0; i < n; ++i) s->cur[beams[i]] ^= 1;
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        s->lat_ms.push_back(dt * 1e3);
+        s->busy_s += dt;
+        s->samples += (double)n * (double)s->chunk;
     });
 }
 
@@ -165,6 +230,7 @@ dedisp_error dedisp_amd_stream_reset(dedisp_amd_stream s)
         std::fill(s->cur.begin(), s->cur.end(), 0);
         s->lat_ms.clear();
         s->busy_s = 0;
+        s->samples = 0;
     });
 }
 
@@ -187,7 +253,7 @@ dedisp_error dedisp_amd_stream_get_stats(const dedisp_amd_stream s, dedisp_amd_s
     st->p50_ms = pct(0.50);
     st->p99_ms = pct(0.99);
     st->max_ms = v.back();
-    st->samples_per_s = s->busy_s > 0 ? (double)v.size() * (double)s->chunk / s->busy_s : 0;
+    st->samples_per_s = s->busy_s > 0 ? s->samples / s->busy_s : 0;
     return DEDISP_NO_ERROR;
 }
 

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 #include "internal.hpp"
 #include "kernels.h"
@@ -48,19 +49,29 @@ struct TDDState {
     DeviceBuffer d_brows, d_bin, d_bout, d_beams, d_tile_beam;
     std::vector<dd_tdd_beam> h_beams;
     std::vector<uint32_t> h_tile_beam;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<uint64_t> graph_key;
 
-    void drop_graph()
-    {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
-        graph_key.clear();
-    }
-    ~TDDState() { drop_graph(); }
+    // Captured device-pointer batches, the least recently used replaced first.
+    // Each keeps its own beam tables, so callers alternating between a few
+    // buffer sets (a stream's double-buffered histories) replay every one.
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        DeviceBuffer d_beams, d_tile_beam;
+        uint64_t used = 0;
+        void drop()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            key.clear();
+        }
+        ~Graph() { drop(); }
+    };
+    static constexpr size_t kGraphs = 4;
+    std::vector<std::unique_ptr<Graph>> graphs;
+    uint64_t graph_clock = 0;
 };
 
 namespace {
@@ -464,8 +475,6 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
         st.d_bin.reserve(in_total);
         st.d_bout.reserve(out_total);
     }
-    st.d_beams.reserve(nbeams * sizeof(dd_tdd_beam));
-    st.d_tile_beam.reserve(ntiles * 4);
     uint8_t* const bout = device_ptrs ? nullptr : st.d_bout.as<uint8_t>();
     const uint8_t* const bin = device_ptrs ? nullptr : st.d_bin.as<uint8_t>();
 
@@ -473,16 +482,39 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
                                  st.tiling.samples_per_lane, st.tiling.dms_per_wave, st.tiling.waves,
                                  st.tiling.chunk_channels, st.tiling.window_words,
                                  (uint64_t)st.d_code.ptr, (uint64_t)st.d_chan.ptr, (uint64_t)st.d_wstart.ptr,
-                                 (uint64_t)st.d_brows.ptr, (uint64_t)st.d_beams.ptr, (uint64_t)st.d_tile_beam.ptr};
+                                 (uint64_t)st.d_brows.ptr};
     for (size_t b = 0; b < nbeams; ++b) {
         key.push_back(nsamps[b]);
         key.push_back((uint64_t)in[b]);
         key.push_back((uint64_t)out[b]);
     }
     const dd_tdd_epilogue ep = make_epilogue(nbits, out_nbits, m_nchans);
-    const bool replay = device_ptrs && st.graph_exec && st.graph_key == key;
+    // device pointers: the cached capture of this batch, else the least
+    // recently used slot for a new one
+    TDDState::Graph* ge = nullptr;
+    bool replay = false;
+    if (device_ptrs) {
+        for (auto& g : st.graphs)
+            if (g->exec && g->key == key) {
+                ge = g.get();
+                replay = true;
+                break;
+            }
+        if (!ge) {
+            if (st.graphs.size() < TDDState::kGraphs) st.graphs.emplace_back(new TDDState::Graph);
+            ge = std::min_element(st.graphs.begin(), st.graphs.end(),
+                                  [](const auto& a, const auto& b) { return a->used < b->used; })
+                     ->get();
+            ge->drop();
+        }
+        ge->used = ++st.graph_clock;
+    }
+    DeviceBuffer& d_beams = ge ? ge->d_beams : st.d_beams;
+    DeviceBuffer& d_tile_beam = ge ? ge->d_tile_beam : st.d_tile_beam;
 
     if (!replay) {
+        d_beams.reserve(nbeams * sizeof(dd_tdd_beam));
+        d_tile_beam.reserve(ntiles * 4);
         st.h_beams.resize(nbeams);
         st.h_tile_beam.clear();
         st.h_tile_beam.reserve(ntiles);
@@ -498,9 +530,9 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
             st.h_tile_beam.insert(st.h_tile_beam.end(), nt[b], (uint32_t)b);
             t0 += nt[b];
         }
-        DD_HIP(hipMemcpyAsync(st.d_beams.ptr, st.h_beams.data(), nbeams * sizeof(dd_tdd_beam),
+        DD_HIP(hipMemcpyAsync(d_beams.ptr, st.h_beams.data(), nbeams * sizeof(dd_tdd_beam),
                               hipMemcpyHostToDevice, stream));
-        DD_HIP(hipMemcpyAsync(st.d_tile_beam.ptr, st.h_tile_beam.data(), ntiles * 4,
+        DD_HIP(hipMemcpyAsync(d_tile_beam.ptr, st.h_tile_beam.data(), ntiles * 4,
                               hipMemcpyHostToDevice, stream));
         DD_HIP(hipStreamSynchronize(stream));
     }
@@ -515,7 +547,7 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
         }
         if (timed) ctx.timer.end(StageTimer::UNPACK, stream);
         if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-        DD_HIP(dd_tdd_dedisperse_lds_batch(st.d_beams.as<dd_tdd_beam>(), st.d_tile_beam.as<uint32_t>(),
+        DD_HIP(dd_tdd_dedisperse_lds_batch(d_beams.as<dd_tdd_beam>(), d_tile_beam.as<uint32_t>(),
                                            ntiles, nbits, st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
                                            st.d_wstart.as<uint32_t>(), st.nact, m_dm_count, ep,
                                            st.tiling, stream));
@@ -525,7 +557,6 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
     if (device_ptrs) {
         if (!replay) {
             // capture the kernels once (no copies or fills inside the graph)
-            st.drop_graph();
             DD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
                 launches(false);
@@ -535,12 +566,12 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
                 if (g) (void)hipGraphDestroy(g);
                 throw;
             }
-            DD_HIP(hipStreamEndCapture(stream, &st.graph));
-            DD_HIP(hipGraphInstantiate(&st.graph_exec, st.graph, nullptr, nullptr, 0));
-            st.graph_key = key;
+            DD_HIP(hipStreamEndCapture(stream, &ge->graph));
+            DD_HIP(hipGraphInstantiate(&ge->exec, ge->graph, nullptr, nullptr, 0));
+            ge->key = key;
         }
         ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-        DD_HIP(hipGraphLaunch(st.graph_exec, stream));
+        DD_HIP(hipGraphLaunch(ge->exec, stream));
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
     } else {
         ctx.timer.begin(StageTimer::H2D, stream);

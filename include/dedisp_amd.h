@@ -103,8 +103,8 @@ const char* dedisp_amd_version(void);
 typedef struct dedisp_amd_stream_struct* dedisp_amd_stream;
 
 typedef struct {
-    unsigned long pushes;       /* pushes since creation or the last reset */
-    double        mean_ms;      /* wall time per push, host-synchronous */
+    unsigned long pushes;       /* push calls since creation or the last reset */
+    double        mean_ms;      /* wall time per push call, host-synchronous */
     double        p50_ms;
     double        p99_ms;
     double        max_ms;
@@ -127,6 +127,14 @@ dedisp_error dedisp_amd_stream_create(dedisp_amd_stream* stream, dedisp_plan pla
 dedisp_error dedisp_amd_stream_push(dedisp_amd_stream stream, dedisp_size beam,
                                     const dedisp_byte* chunk, dedisp_byte* out,
                                     unsigned flags);
+/* One chunk for each of n distinct beams (beams[i] < nbeams) in one batched
+ * execution (dedisp_amd_execute_batch over the beams' [history | chunk]
+ * buffers): the outputs equal n single pushes. The statistics record one
+ * latency per call (the time until every output of the call is written).
+ * A repeated beam index is rejected with DEDISP_INVALID_POINTER. */
+dedisp_error dedisp_amd_stream_push_batch(dedisp_amd_stream stream, dedisp_size n,
+                                          const dedisp_size* beams, const dedisp_byte* const* chunks,
+                                          dedisp_byte* const* outs, unsigned flags);
 /* Zeroes every beam's history and the statistics. */
 dedisp_error dedisp_amd_stream_reset(dedisp_amd_stream stream);
 dedisp_error dedisp_amd_stream_get_stats(const dedisp_amd_stream stream,

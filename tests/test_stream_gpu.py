@@ -146,3 +146,61 @@ def test_stream_cfg5_geometry(dd, orc):
         for i in (1, 2999):
             ref = orc.tdd(nch, delays, None, dms, chunk * npush, hist, data, 8, 32, first_dm=i, dm_count=1)
             assert np.array_equal(got[i], ref[0]), (beam, i)
+
+
+@pytest.mark.parametrize("impl", ["TDD", "FDD_BF16"])
+def test_stream_push_batch_matches_single_pushes(dd, impl):
+    """Batched pushes (one chunk of several beams per call, through the plan's
+    batched execution) give exactly the outputs of single pushes. The beam
+    subsets differ between calls and repeat, so the TDD batch replays captured
+    graphs over both of each beam's alternating history buffers."""
+    import torch
+    nch = 128
+    if impl == "TDD":
+        plan = dd.Plan(nch, 64e-6, 1581.8, -0.78125, impl=dd.TDD)
+    else:
+        plan = dd.Plan(nch, 64e-6, 1581.8, -0.78125, impl=dd.FDD)
+        plan.fdd_set_precision(dd.FDD_BF16)
+    plan.set_dm_list(np.linspace(0, 120, 45, dtype=f32))
+    if impl != "TDD":
+        plan.fdd_set_segmented(True, 8192)
+    chunk, nb = 3000, 4
+    ref_st = dd.Stream(plan, nbeams=nb, chunk_nsamps=chunk, in_nbits=8, out_nbits=32)
+    bat_st = dd.Stream(plan, nbeams=nb, chunk_nsamps=chunk, in_nbits=8, out_nbits=32)
+    rng = np.random.default_rng(17)
+    ndm = plan.dm_count
+    calls = [[0, 1, 2, 3], [2, 0], [1, 3, 0, 2], [0, 1, 2, 3], [3], [0, 1, 2, 3], [0, 1, 2, 3]]
+    d_out = [torch.empty((ndm, chunk), dtype=torch.float32, device="cuda") for _ in range(nb)]
+    one = torch.empty((ndm, chunk), dtype=torch.float32, device="cuda")
+    for n, beams in enumerate(calls):
+        chunks = [torch.from_numpy(np.clip(np.rint(rng.normal(128, 20, size=(chunk, nch))), 0, 255)
+                                   .astype(np.uint8)).cuda() for _ in beams]
+        torch.cuda.synchronize()
+        assert bat_st.push_batch_raw(beams, chunks, [d_out[b] for b in beams], dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        for b, c in zip(beams, chunks):
+            assert ref_st.push_raw(b, c, one, dd.DEVICE_POINTERS) == 0
+            torch.cuda.synchronize()
+            assert torch.equal(d_out[b], one), (n, b)
+    s = bat_st.stats()
+    assert s["pushes"] == len(calls)
+    assert s["samples_per_s"] > 0
+
+
+def test_stream_push_batch_host_and_errors(dd):
+    nch = 64
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.TDD)
+    plan.set_dm_list(np.linspace(0, 40, 11, dtype=f32))
+    st = dd.Stream(plan, nbeams=3, chunk_nsamps=1000, in_nbits=8, out_nbits=16)
+    ref = dd.Stream(plan, nbeams=3, chunk_nsamps=1000, in_nbits=8, out_nbits=16)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        chunks = [rng.integers(0, 256, size=(1000, nch), dtype=np.uint8) for _ in range(3)]
+        outs = st.push_batch([2, 0, 1], [chunks[2], chunks[0], chunks[1]])
+        for b, o in zip([2, 0, 1], outs):
+            assert np.array_equal(o, ref.push(b, chunks[b])), b
+    buf = np.zeros((1000, nch), np.uint8)
+    out = np.zeros((11, 1000), np.uint16)
+    assert st.push_batch_raw([1, 1], [buf, buf], [out, out]) == dd.DEDISP_INVALID_POINTER  # repeated beam
+    assert st.push_batch_raw([3], [buf], [out]) == dd.DEDISP_INVALID_POINTER               # out of range
+    assert st.push_batch_raw([], [], []) == 0
