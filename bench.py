@@ -238,7 +238,8 @@ def batch_leg(torch, dd, steps: int = 2) -> dict:
     launch for all beams, replayed as one HIP graph (device pointers, 8-bit
     output); then FDD-BF16 on the first 8 beams (the overlap-save chunks of the
     8 beams concatenated into one forward transform, contraction and inverse
-    transform sequence; float output). Inputs and outputs stay in HBM."""
+    transform sequence, replayed as one HIP graph; float output). Inputs and
+    outputs stay in HBM."""
     nb, nch, dt, f0, df = 64, 512, 81.92e-6, 1500.0, -0.78125
     rng = np.random.default_rng(4)
     lengths = [int(2 ** rng.uniform(17, 20)) for _ in range(nb)]
@@ -311,7 +312,7 @@ def batch_leg(torch, dd, steps: int = 2) -> dict:
     del one
     out["fdd_bf16_8beams"] = {"value": 1024 * sum(lengths[b] - md for b in sub) / el, "unit": "DM*samples/s",
                               "ms_per_batch": 1e3 * el, "nfft": ftune["nfft"],
-                              "launch": "8 transposes + 1 forward, 1 contraction, 1 inverse launch per DM batch",
+                              "launch": "one HIP graph: 8 transposes + 1 forward, 1 contraction, 1 inverse launch per DM batch",
                               "parity": fparity}
     del f_out, plan, d_in
     torch.cuda.empty_cache()

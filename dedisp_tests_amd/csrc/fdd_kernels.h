@@ -12,6 +12,10 @@ extern "C" {
 // nfft values the on-chip transform engine handles.
 int dd_fdd_fft_supported(size_t nfft);
 
+// Zeroes bytes at p (both multiples of 16) with a kernel: the form that may
+// be recorded in a captured graph.
+hipError_t dd_zero_fill(void* p, size_t bytes, hipStream_t stream);
+
 // One overlap-save chunk of one beam, for transforms over the chunks of
 // several beams at once (all beams share nfft, the channels and the DM list;
 // the chunk's good length is the launch's). Forward: the beam's channel rows

@@ -37,7 +37,29 @@ struct FDDState {
     DeviceBuffer d_Z;       // [nchan padded to 16][nchunk][Xp] spectra, bins 0..nfft/2
     DeviceBuffer d_Y;       // [dm batch][nchunk][Yp] dedispersed spectra
     DeviceBuffer d_out;     // staged output (host-pointer mode)
-    DeviceBuffer d_chunks;  // [chunks of a batch] dd_fdd_chunk
+    DeviceBuffer d_chunks;  // [chunks of a batch] dd_fdd_chunk (host-pointer batches)
+
+    // Captured device-pointer batches, the least recently used replaced first;
+    // each keeps its own chunk table.
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        DeviceBuffer d_chunks;
+        uint64_t used = 0;
+        void drop()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            key.clear();
+        }
+        ~Graph() { drop(); }
+    };
+    static constexpr size_t kGraphs = 4;
+    std::vector<std::unique_ptr<Graph>> graphs;
+    uint64_t graph_clock = 0;
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
     DeviceBuffer d_q16;     // [ndm][qpitch] uint16 phase steps d*65536/nfft mod 65536
     DeviceBuffer d_tab;     // [512] e^{2 pi i 256a/65536}, e^{2 pi i b/65536}
@@ -570,63 +592,88 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
     st.d_rows.reserve(g_rows * 4);
     st.d_Z.reserve(nchan_pad * g_chunks * Xp * 8);
     st.d_Y.reserve(dm_batch * g_chunks * Yp * 8);
-    st.d_chunks.reserve(total_chunks * sizeof(dd_fdd_chunk));
     if (!device_ptrs) {
         st.d_in.reserve(g_in + 64);
         st.d_out.reserve(g_out * 4);
     }
 
-    // ---- chunk tables (one upload for the batch) ----
-    std::vector<dd_fdd_chunk> tab(total_chunks);
-    for (const Group& g : groups) {
-        size_t i = g.chunk0;
-        for (size_t b = g.b0; b < g.b1; ++b)
-            for (size_t j = 0; j < nch[b]; ++j, ++i) {
-                dd_fdd_chunk& e = tab[i];
-                e.rows = st.d_rows.as<uint8_t>() + roff[b] * 4;
-                e.out = device_ptrs ? reinterpret_cast<float*>(out[b]) : st.d_out.as<float>() + ooff[b];
-                e.row_bytes = rw[b] * 4;
-                e.out_stride = nsc[b];
-                e.t0 = (uint32_t)(j * good);
-                e.nvalid = (uint32_t)nsamps[b];
-                e.nsamps_out = (uint32_t)nsc[b];
-                e.reserved = 0;
-            }
+    // device pointers: a captured graph per batch (lengths, pointers, segment
+    // length, precision, tables and buffers), replayed while they stay the same
+    std::vector<uint64_t> key = {nbeams, in_nbits, nfft, (uint64_t)st.precision, valu ? 1u : 0u,
+                                 m_tables_version, max_chunks, dm_batch, __builtin_bit_cast(uint32_t, xscale),
+                                 (uint64_t)st.d_rows.ptr, (uint64_t)st.d_Z.ptr, (uint64_t)st.d_Y.ptr,
+                                 (uint64_t)st.d_q16.ptr, (uint64_t)st.d_tab.ptr, (uint64_t)st.d_ph.ptr,
+                                 (uint64_t)st.d_scale.ptr, (uint64_t)st.d_delays.ptr};
+    for (size_t b = 0; b < nbeams; ++b) {
+        key.push_back(nsamps[b]);
+        key.push_back((uint64_t)in[b]);
+        key.push_back((uint64_t)out[b]);
     }
-    DD_HIP(hipMemcpyAsync(st.d_chunks.ptr, tab.data(), tab.size() * sizeof(dd_fdd_chunk),
-                          hipMemcpyHostToDevice, stream));
-    DD_HIP(hipStreamSynchronize(stream));
-    const dd_fdd_chunk* d_tab = st.d_chunks.as<dd_fdd_chunk>();
-
-    for (const Group& g : groups) {
-        if (g.nchunk == 0) continue;
-        const size_t nc = g.nchunk;
-        if (!device_ptrs) {
-            ctx.timer.begin(StageTimer::H2D, stream);
-            for (size_t b = g.b0; b < g.b1; ++b)
-                if (nch[b])
-                    DD_HIP(hipMemcpyAsync(st.d_in.as<uint8_t>() + ioff[b], in[b], nsamps[b] * in_row,
-                                          hipMemcpyHostToDevice, stream));
-            ctx.timer.end(StageTimer::H2D, stream);
+    FDDState::Graph* ge = nullptr;
+    bool replay = false;
+    if (device_ptrs) {
+        for (auto& g : st.graphs)
+            if (g->exec && g->key == key) {
+                ge = g.get();
+                replay = true;
+                break;
+            }
+        if (!ge) {
+            if (st.graphs.size() < FDDState::kGraphs) st.graphs.emplace_back(new FDDState::Graph);
+            ge = std::min_element(st.graphs.begin(), st.graphs.end(),
+                                  [](const auto& x, const auto& y) { return x->used < y->used; })
+                     ->get();
+            ge->drop();
         }
+        ge->used = ++st.graph_clock;
+    }
+    DeviceBuffer& d_chunks = ge ? ge->d_chunks : st.d_chunks;
+
+    // ---- chunk tables (one upload for the batch) ----
+    if (!replay) {
+        d_chunks.reserve(total_chunks * sizeof(dd_fdd_chunk));
+        std::vector<dd_fdd_chunk> tab(total_chunks);
+        for (const Group& g : groups) {
+            size_t i = g.chunk0;
+            for (size_t b = g.b0; b < g.b1; ++b)
+                for (size_t j = 0; j < nch[b]; ++j, ++i) {
+                    dd_fdd_chunk& e = tab[i];
+                    e.rows = st.d_rows.as<uint8_t>() + roff[b] * 4;
+                    e.out = device_ptrs ? reinterpret_cast<float*>(out[b]) : st.d_out.as<float>() + ooff[b];
+                    e.row_bytes = rw[b] * 4;
+                    e.out_stride = nsc[b];
+                    e.t0 = (uint32_t)(j * good);
+                    e.nvalid = (uint32_t)nsamps[b];
+                    e.nsamps_out = (uint32_t)nsc[b];
+                    e.reserved = 0;
+                }
+        }
+        DD_HIP(hipMemcpyAsync(d_chunks.ptr, tab.data(), tab.size() * sizeof(dd_fdd_chunk),
+                              hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+    }
+    const dd_fdd_chunk* d_tab = d_chunks.as<dd_fdd_chunk>();
+
+    // the kernels of one group (no copies: they may be captured)
+    auto group_kernels = [&](const Group& g, bool timed) {
+        const size_t nc = g.nchunk;
         if (nchan_pad > nchan)
-            DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nc * Xp, 0, (nchan_pad - nchan) * nc * Xp * 8,
-                                  stream));
-        ctx.timer.begin(StageTimer::UNPACK, stream);
+            DD_HIP(dd_zero_fill(st.d_Z.as<float2>() + nchan * nc * Xp, (nchan_pad - nchan) * nc * Xp * 8, stream));
+        if (timed) ctx.timer.begin(StageTimer::UNPACK, stream);
         for (size_t b = g.b0; b < g.b1; ++b) {
             if (!nch[b]) continue;
             const uint8_t* src = device_ptrs ? in[b] : st.d_in.as<uint8_t>() + ioff[b];
             DD_HIP(dd_tdd_transpose(src, nsamps[b], in_row, (unsigned)in_nbits, nchan,
                                     st.d_rows.as<uint32_t>() + roff[b], rw[b], stream));
         }
-        ctx.timer.end(StageTimer::UNPACK, stream);
-        ctx.timer.begin(StageTimer::FFT, stream);
+        if (timed) ctx.timer.end(StageTimer::UNPACK, stream);
+        if (timed) ctx.timer.begin(StageTimer::FFT, stream);
         DD_HIP(dd_fdd_forward_chunks(d_tab + g.chunk0, nc, st.d_scale.as<float>(), ph, nfft,
                                      st.d_Z.as<float2>(), nchan, Xp, (unsigned)in_nbits, stream));
-        ctx.timer.end(StageTimer::FFT, stream);
+        if (timed) ctx.timer.end(StageTimer::FFT, stream);
         for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
             const size_t nb = std::min(dm_batch, ndm - b0);
-            ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+            if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
             if (valu)
                 DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, st.d_delays.as<int32_t>() + b0 * nchan, nb,
                                             nchan, nc, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
@@ -635,13 +682,43 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
                                             st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan, nc,
                                             st.d_tab.as<float2>(), nfft, xscale, st.d_Y.as<float2>(), Yp,
                                             stream));
-            ctx.timer.end(StageTimer::DEDISPERSE, stream);
-            ctx.timer.begin(StageTimer::IFFT, stream);
+            if (timed) ctx.timer.end(StageTimer::DEDISPERSE, stream);
+            if (timed) ctx.timer.begin(StageTimer::IFFT, stream);
             DD_HIP(dd_fdd_inverse_chunks(st.d_Y.as<float2>(), nb, b0, d_tab + g.chunk0, nc, Yp, ph, nfft, good,
                                          stream));
-            ctx.timer.end(StageTimer::IFFT, stream);
+            if (timed) ctx.timer.end(StageTimer::IFFT, stream);
         }
-        if (!device_ptrs) {
+    };
+
+    if (device_ptrs) {
+        if (!replay) {
+            DD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                for (const Group& g : groups)
+                    if (g.nchunk) group_kernels(g, false);
+            } catch (...) {
+                hipGraph_t gr = nullptr;
+                (void)hipStreamEndCapture(stream, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            DD_HIP(hipStreamEndCapture(stream, &ge->graph));
+            DD_HIP(hipGraphInstantiate(&ge->exec, ge->graph, nullptr, nullptr, 0));
+            ge->key = key;
+        }
+        ctx.timer.begin(StageTimer::DEDISPERSE, stream);
+        DD_HIP(hipGraphLaunch(ge->exec, stream));
+        ctx.timer.end(StageTimer::DEDISPERSE, stream);
+    } else {
+        for (const Group& g : groups) {
+            if (g.nchunk == 0) continue;
+            ctx.timer.begin(StageTimer::H2D, stream);
+            for (size_t b = g.b0; b < g.b1; ++b)
+                if (nch[b])
+                    DD_HIP(hipMemcpyAsync(st.d_in.as<uint8_t>() + ioff[b], in[b], nsamps[b] * in_row,
+                                          hipMemcpyHostToDevice, stream));
+            ctx.timer.end(StageTimer::H2D, stream);
+            group_kernels(g, true);
             ctx.timer.begin(StageTimer::D2H, stream);
             for (size_t b = g.b0; b < g.b1; ++b)
                 if (nch[b])

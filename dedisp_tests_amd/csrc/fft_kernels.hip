@@ -22,6 +22,8 @@
 // double precision on the host: W_M^j = conj(ph[2j]).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fdd_kernels.h"
 
 namespace {
@@ -559,6 +561,23 @@ extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t n
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fourier_step_c_k, dim3((unsigned)((L + 255) / 256), (unsigned)nrows), dim3(256),
                        0, stream, Z, z_stride, L, twN, out, out_stride);
+    return hipGetLastError();
+}
+
+// Zero fill by a kernel (16-byte stores), usable inside captured graphs.
+__global__ __launch_bounds__(256) void zero_fill_k(uint4* __restrict__ p, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+extern "C" hipError_t dd_zero_fill(void* p, size_t bytes, hipStream_t stream)
+{
+    if (((uintptr_t)p | bytes) & 15) return hipErrorInvalidValue;
+    const size_t n16 = bytes / 16;
+    if (!n16) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<size_t>((n16 + 255) / 256, 2048);
+    hipLaunchKernelGGL(zero_fill_k, dim3(grid), dim3(256), 0, stream, static_cast<uint4*>(p), n16);
     return hipGetLastError();
 }
 

@@ -264,3 +264,36 @@ def test_batch_uneven_beams_512ch_2bit(dd, orc):
     for r in (3, 1000):
         ref = orc.tdd(nch, delays, None, dms, lengths[2], md, host[2], nbits, 8, first_dm=r, dm_count=1)
         assert np.array_equal(got[r], ref[0]), r
+
+
+def test_fdd_batch_graph_replays(dd):
+    """Device-pointer FDD batches are captured once and replayed: new input
+    contents each time, single executes (which reuse the plan's work buffers)
+    and unrelated device work in between, 100 channels so the graph also
+    clears the padded spectrum rows; every replay equals single executes."""
+    import torch
+    nch = 100
+    plan = dd.Plan(nch, 64e-6, 1581.8, -1.5, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    plan.fdd_set_precision(dd.FDD_BF16)
+    plan.set_dm_list(np.linspace(0, 90, 40, dtype=f32))
+    plan.fdd_set_segmented(True, 8192)
+    md, ndm = plan.max_delay, plan.dm_count
+    rng = np.random.default_rng(23)
+    lengths = [9000, 21000, 5000]
+    d_in = [torch.zeros((n, nch), dtype=torch.uint8, device="cuda") for n in lengths]
+    d_out = [torch.empty((ndm, n - md), dtype=torch.float32, device="cuda") for n in lengths]
+    one = [torch.empty_like(o) for o in d_out]
+    for rep in range(4):
+        for t, n in zip(d_in, lengths):
+            t.copy_(torch.from_numpy(_u8_beams(rng, (n,), nch)[0]))
+        torch.cuda.synchronize()
+        assert plan.execute_batch_raw(lengths, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        x = torch.randn(1 << 20, device="cuda")
+        (x * 2).sum().item()
+        for t, o, n in zip(d_in, one, lengths):
+            assert plan.execute_raw(n, t, 8, o, 32, dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        for i in range(len(lengths)):
+            assert torch.equal(d_out[i], one[i]), (rep, i)
