@@ -60,6 +60,13 @@ struct FDDState {
     static constexpr size_t kGraphs = 4;
     std::vector<std::unique_ptr<Graph>> graphs;
     uint64_t graph_clock = 0;
+
+    // end of each DM batch (host-pointer executes)
+    std::vector<hipEvent_t> blk_events;
+    ~FDDState()
+    {
+        for (hipEvent_t e : blk_events) (void)hipEventDestroy(e);
+    }
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
     DeviceBuffer d_q16;     // [ndm][qpitch] uint16 phase steps d*65536/nfft mod 65536
     DeviceBuffer d_tab;     // [512] e^{2 pi i 256a/65536}, e^{2 pi i b/65536}
@@ -379,7 +386,7 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     } else {
         st.d_in.reserve(nsamps * in_row + 64);
         ctx.timer.begin(StageTimer::H2D, stream);
-        DD_HIP(hipMemcpyAsync(st.d_in.ptr, in, nsamps * in_row, hipMemcpyHostToDevice, stream));
+        ctx.io.upload(stream, st.d_in.as<uint8_t>(), in_row, in, in_row, in_row, nsamps);
         ctx.timer.end(StageTimer::H2D, stream);
         d_in = st.d_in.as<uint8_t>();
     }
@@ -432,7 +439,19 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
 
     // ---- per DM batch: contraction, inverse transforms ----
     const size_t per_dm = nchunk * Yp * 8 + (fourier ? 2 * Nf * 4 : 0);
-    const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    if (!device_ptrs) {
+        // host outputs: DM batches of about 1 GB of output (whole 256-DM tiles
+        // of the contraction), so the download overlaps the later batches
+        const size_t want = round_up(std::max<size_t>(1, (size_t(1) << 30) / (out_floats * 4)), 256);
+        dm_batch = std::min(dm_batch, want);
+        const size_t nbat = div_up(ndm, dm_batch);
+        while (st.blk_events.size() < nbat) {
+            hipEvent_t e = nullptr;
+            DD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            st.blk_events.push_back(e);
+        }
+    }
     st.d_Y.reserve(dm_batch * nchunk * Yp * 8);
     if (fourier) {
         st.d_S.reserve(dm_batch * Nf * 4);
@@ -473,12 +492,20 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
                                     reinterpret_cast<float2*>(d_out + b0 * out_floats), Nf / 2, stream));
         }
         ctx.timer.end(StageTimer::IFFT, stream);
+        if (!device_ptrs) DD_HIP(hipEventRecord(st.blk_events[b0 / dm_batch], stream));
     }
 
     if (!device_ptrs) {
-        ctx.timer.begin(StageTimer::D2H, stream);
-        DD_HIP(hipMemcpyAsync(out, d_out, ndm * out_floats * 4, hipMemcpyDeviceToHost, stream));
-        ctx.timer.end(StageTimer::D2H, stream);
+        hipStream_t d2h = ctx.d2h;
+        size_t waited = 0;  // DM batches the copy stream already waits for
+        ctx.timer.begin(StageTimer::D2H, d2h);
+        ctx.io.download(d2h, out, out_floats * 4, reinterpret_cast<const uint8_t*>(d_out), out_floats * 4,
+                        out_floats * 4, ndm, [&](size_t rows) {
+                            const size_t need = div_up(rows, dm_batch);
+                            for (; waited < need; ++waited)
+                                DD_HIP(hipStreamWaitEvent(d2h, st.blk_events[waited], 0));
+                        });
+        ctx.timer.end(StageTimer::D2H, d2h);
     }
     if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
         DD_HIP(hipStreamSynchronize(stream));

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,30 @@ struct PinnedBuffer {
     void release();
 };
 
+// Transfers between the caller's pageable host memory and the device through
+// two page-locked slots (host copies on a worker pool overlap the DMA).
+class HostIO {
+public:
+    HostIO() = default;
+    HostIO(const HostIO&) = delete;
+    HostIO& operator=(const HostIO&) = delete;
+    ~HostIO();
+    // rows x row_bytes from host src (src_stride apart) to device dst
+    // (dst_stride apart), queued on stream; returns when src may be reused
+    void upload(hipStream_t stream, uint8_t* dst, size_t dst_stride, const uint8_t* src, size_t src_stride,
+                size_t row_bytes, size_t rows);
+    // rows x row_bytes from device src to host dst; before the copy of rows
+    // below r is queued on stream, ready(r) is called (to make stream wait for
+    // the work that writes them). Returns when dst holds every row.
+    void download(hipStream_t stream, uint8_t* dst, size_t dst_stride, const uint8_t* src, size_t src_stride,
+                  size_t row_bytes, size_t rows, const std::function<void(size_t)>& ready);
+
+private:
+    void prepare();
+    PinnedBuffer m_slot[2];
+    hipEvent_t m_ev[2] = {nullptr, nullptr};
+};
+
 // Records event pairs around launches while profiling is on and sums them
 // into a dedisp_amd_profile after the stream has drained.
 class StageTimer {
@@ -78,6 +103,7 @@ struct GPUContext {
     DeviceBuffer d_dm_list;      // float[ndm]
     DeviceBuffer d_delay_table;  // float[nchans]
     DeviceBuffer d_killmask;     // int[nchans]
+    HostIO io;                   // host-pointer transfers
     bool profiling = false;
     StageTimer timer;
     dedisp_amd_profile profile{};

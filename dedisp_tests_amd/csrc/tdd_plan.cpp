@@ -72,6 +72,13 @@ struct TDDState {
     static constexpr size_t kGraphs = 4;
     std::vector<std::unique_ptr<Graph>> graphs;
     uint64_t graph_clock = 0;
+
+    // end of each DM block's channel sum (host-pointer executes)
+    std::vector<hipEvent_t> blk_events;
+    ~TDDState()
+    {
+        for (hipEvent_t e : blk_events) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -327,22 +334,23 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
     // ---- input ----
     const size_t row_bytes = (m_nchans * nbits + 7) / 8;
     const uint8_t* d_in;
+    size_t d_in_stride = in_stride;
     if (device_ptrs) {
         d_in = in;
     } else {
-        const size_t width = std::min(in_stride, row_bytes);
-        st.d_in.reserve(nsamps * in_stride + 64);
+        // rows packed on the device; the host rows go through page-locked slots
+        st.d_in.reserve(nsamps * row_bytes + 64);
         ctx.timer.begin(StageTimer::H2D, stream);
-        DD_HIP(hipMemcpy2DAsync(st.d_in.ptr, in_stride, in, in_stride, width, nsamps,
-                                hipMemcpyHostToDevice, stream));
+        ctx.io.upload(stream, st.d_in.as<uint8_t>(), row_bytes, in, in_stride, row_bytes, nsamps);
         ctx.timer.end(StageTimer::H2D, stream);
         d_in = st.d_in.as<uint8_t>();
+        d_in_stride = row_bytes;
     }
 
     // ---- transpose ----
     st.d_rows.reserve(m_nchans * st.row_words * 4);
     ctx.timer.begin(StageTimer::UNPACK, stream);
-    DD_HIP(dd_tdd_transpose(d_in, nsamps, in_stride, nbits, m_nchans, st.d_rows.as<uint32_t>(),
+    DD_HIP(dd_tdd_transpose(d_in, nsamps, d_in_stride, nbits, m_nchans, st.d_rows.as<uint32_t>(),
                             st.row_words, stream));
     ctx.timer.end(StageTimer::UNPACK, stream);
 
@@ -367,26 +375,56 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
         ep.factor = (3.f * 1024.f) / 255.f / 16.f;
         ep.out_nbits = (unsigned)out_nbits;
     }
+    // Host outputs: the channel sum runs in DM blocks of about 256 MB of output
+    // (whole kernel DM blocks), each followed by an event, so the download of
+    // the first blocks overlaps the sums of the later ones.
+    size_t blk = dm_count;
+    if (!device_ptrs && st.fast) {
+        const size_t grp = (size_t)st.tiling.waves * st.tiling.dms_per_wave;
+        const size_t want = std::max<size_t>(1, (size_t(256) << 20) / (nsc * out_bytes_per_sample));
+        blk = std::min(dm_count, round_up(want, grp));
+    }
+    const size_t nblk = div_up(dm_count, blk);
+    if (!device_ptrs) {
+        while (st.blk_events.size() < nblk) {
+            hipEvent_t e = nullptr;
+            DD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            st.blk_events.push_back(e);
+        }
+    }
     ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-    if (st.fast) {
-        DD_HIP(dd_tdd_dedisperse_lds(st.d_rows.as<uint32_t>(), st.row_words, nbits,
-                                     st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
-                                     st.d_wstart.as<uint32_t>(), st.nact, dm_count, nsc, d_out,
-                                     d_out_stride, ep, st.tiling, stream));
-    } else {
-        DD_HIP(dd_tdd_dedisperse_generic(st.d_rows.as<uint32_t>(), st.row_words, nbits,
-                                         st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
-                                         st.d_weight.as<int32_t>(), st.nact, dm_count, nsc,
-                                         d_out, d_out_stride, ep, stream));
+    for (size_t k = 0; k < nblk; ++k) {
+        const size_t b0 = k * blk, nb = std::min(blk, dm_count - b0);
+        if (st.fast) {
+            // the tables of DM block b0 / (waves * dms_per_wave) start at DM b0
+            DD_HIP(dd_tdd_dedisperse_lds(st.d_rows.as<uint32_t>(), st.row_words, nbits,
+                                         st.d_code.as<uint32_t>() + b0 * st.nact, st.d_chan.as<int32_t>(),
+                                         st.d_wstart.as<uint32_t>() +
+                                             b0 / ((size_t)st.tiling.waves * st.tiling.dms_per_wave) * st.nact,
+                                         st.nact, nb, nsc, d_out + b0 * d_out_stride, d_out_stride, ep, st.tiling,
+                                         stream));
+        } else {
+            DD_HIP(dd_tdd_dedisperse_generic(st.d_rows.as<uint32_t>(), st.row_words, nbits,
+                                             st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
+                                             st.d_weight.as<int32_t>(), st.nact, dm_count, nsc,
+                                             d_out, d_out_stride, ep, stream));
+        }
+        if (!device_ptrs) DD_HIP(hipEventRecord(st.blk_events[k], stream));
     }
     ctx.timer.end(StageTimer::DEDISPERSE, stream);
 
     // ---- output ----
     if (!device_ptrs) {
-        ctx.timer.begin(StageTimer::D2H, stream);
-        DD_HIP(hipMemcpy2DAsync(out, out_stride, d_out, d_out_stride, nsc * out_bytes_per_sample,
-                                dm_count, hipMemcpyDeviceToHost, stream));
-        ctx.timer.end(StageTimer::D2H, stream);
+        hipStream_t d2h = ctx.d2h;
+        size_t waited = 0;  // blocks the copy stream already waits for
+        ctx.timer.begin(StageTimer::D2H, d2h);
+        ctx.io.download(d2h, out, out_stride, d_out, d_out_stride, nsc * out_bytes_per_sample, dm_count,
+                        [&](size_t rows) {
+                            const size_t need = div_up(rows, blk);
+                            for (; waited < need; ++waited)
+                                DD_HIP(hipStreamWaitEvent(d2h, st.blk_events[waited], 0));
+                        });
+        ctx.timer.end(StageTimer::D2H, d2h);
     }
     if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
         DD_HIP(hipStreamSynchronize(stream));
