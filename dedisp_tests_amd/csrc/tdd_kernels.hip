@@ -348,56 +348,65 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         }
         __syncthreads();
 
+        // Adds the two channels of a pair whose codes are c (2*DPW of them).
+        auto pair = [&](const uint32_t (&c)[2 * DPW]) {
+            if constexpr (NBITS < 8 && TDD_EARLY_READS) {
+                // all LDS words of the pair first, so their latencies overlap
+                uint32_t wv[2 * DPW][G][2];
+#pragma unroll
+                for (int t = 0; t < 2 * DPW; ++t)
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        const uint32_t* q = window(c[t]) + lane + 64 * k;
+                        wv[t][k][0] = q[0];
+                        wv[t][k][1] = q[1];
+                    }
+#pragma unroll
+                for (int i = 0; i < DPW; ++i)
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        const uint32_t x0 = __builtin_amdgcn_alignbit(wv[i][k][1], wv[i][k][0], c[i]);
+                        const uint32_t x1 = __builtin_amdgcn_alignbit(wv[DPW + i][k][1], wv[DPW + i][k][0], c[DPW + i]);
+#pragma unroll
+                        for (int m = 0; m < P; ++m)
+                            acc8[i][k][m] += ((x0 >> (m * NBITS)) & M8) + ((x1 >> (m * NBITS)) & M8);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < DPW; ++i)
+                    accumulate2(i, window(c[i]), c[i], window(c[DPW + i]), c[DPW + i]);
+            }
+        };
+        // codes of pair jj (scalar loads; LDS and scalar loads share one
+        // counter, so the first wait of a pair covers the prefetch too)
+        auto load_codes = [&](uint32_t (&c)[2 * DPW], uint32_t jj) {
+            const uint32_t* cp = code + (coff + (j0 + jj) * DPW);
+#pragma unroll
+            for (int t = 0; t < 2 * DPW; ++t) c[t] = cp[t];
+        };
+
         for (uint32_t g0 = 0; g0 < cc; g0 += GROUP) {
             const uint32_t g1 = min(cc, g0 + GROUP);
             uint32_t jj = g0;
-            // codes of the first pair; each iteration fetches the next pair's
-            // (clamped to the last pair of the group: in bounds, unused)
-            uint32_t cn[2 * DPW];
-            if (jj + 1 < g1) {
-                const uint32_t* cp = code + (coff + (j0 + jj) * DPW);
-#pragma unroll
-                for (int t = 0; t < 2 * DPW; ++t) cn[t] = cp[t];
+            // two pairs per iteration with two code sets, each fetched one pair
+            // ahead; the last prefetch may read one pair past the group (the
+            // code table is padded, so it stays in bounds and is not used)
+            uint32_t ca[2 * DPW], cb[2 * DPW];
+            if (jj + 1 < g1) load_codes(ca, jj);
+            // the first codes are in registers before the loop (lgkmcnt(0)), so
+            // no path into the loop leaves a scalar load to wait for at its top
+            __builtin_amdgcn_s_waitcnt(0xc07f);
+            for (; jj + 3 < g1; jj += 4) {
+                load_codes(cb, jj + 2);
+                __builtin_amdgcn_sched_barrier(0);
+                pair(ca);
+                load_codes(ca, jj + 4);
+                __builtin_amdgcn_sched_barrier(0);
+                pair(cb);
             }
-            for (; jj + 1 < g1; jj += 2) {
-                uint32_t c[2 * DPW];
-#pragma unroll
-                for (int t = 0; t < 2 * DPW; ++t) c[t] = cn[t];
-                {
-                    // issued before this pair's LDS reads: LDS and scalar loads
-                    // share one counter, so the first wait covers both
-                    const uint32_t jn = (jj + 3 < g1) ? jj + 2 : jj;
-                    const uint32_t* cp = code + (coff + (j0 + jn) * DPW);
-#pragma unroll
-                    for (int t = 0; t < 2 * DPW; ++t) cn[t] = cp[t];
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                if constexpr (NBITS < 8 && TDD_EARLY_READS) {
-                    // all LDS words of the pair first, so their latencies overlap
-                    uint32_t wv[2 * DPW][G][2];
-#pragma unroll
-                    for (int t = 0; t < 2 * DPW; ++t)
-#pragma unroll
-                        for (int k = 0; k < G; ++k) {
-                            const uint32_t* q = window(c[t]) + lane + 64 * k;
-                            wv[t][k][0] = q[0];
-                            wv[t][k][1] = q[1];
-                        }
-#pragma unroll
-                    for (int i = 0; i < DPW; ++i)
-#pragma unroll
-                        for (int k = 0; k < G; ++k) {
-                            const uint32_t x0 = __builtin_amdgcn_alignbit(wv[i][k][1], wv[i][k][0], c[i]);
-                            const uint32_t x1 = __builtin_amdgcn_alignbit(wv[DPW + i][k][1], wv[DPW + i][k][0], c[DPW + i]);
-#pragma unroll
-                            for (int m = 0; m < P; ++m)
-                                acc8[i][k][m] += ((x0 >> (m * NBITS)) & M8) + ((x1 >> (m * NBITS)) & M8);
-                        }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < DPW; ++i)
-                        accumulate2(i, window(c[i]), c[i], window(c[DPW + i]), c[DPW + i]);
-                }
+            if (jj + 1 < g1) {
+                pair(ca);
+                jj += 2;
             }
             if (jj < g1) {
                 const uint32_t* cp = code + (coff + (j0 + jj) * DPW);

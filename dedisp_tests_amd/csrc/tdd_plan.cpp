@@ -231,7 +231,9 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
             DD_HIP(hipMemcpyAsync(st.d_weight.ptr, weight.data(), nact * 4, hipMemcpyHostToDevice, stream));
         }
         const std::vector<uint32_t>& tab = fast ? code : d;
-        st.d_code.reserve(std::max<size_t>(tab.size(), 1) * 4);
+        // 64 spare entries: the channel-sum kernel may prefetch one pair of
+        // codes past its last channel
+        st.d_code.reserve((tab.size() + 64) * 4);
         if (!tab.empty())
             DD_HIP(hipMemcpyAsync(st.d_code.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
         if (fast) {

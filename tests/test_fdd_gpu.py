@@ -248,3 +248,26 @@ def test_fdd_subbyte_input_vs_oracle(dd, orc, nbits):
         out = plan.execute(packed, nbits, 32)
         ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, s.astype(np.uint8), nbits=nbits)
         assert rel_l2(out, ref) < REL_FP32
+
+
+def test_fdd_host_pointers_many_dm_batches(dd):
+    """Host-pointer FDD execute whose output spans several 256-DM batches of
+    about 1 GB (downloads overlapped with later batches): identical to the
+    device-pointer execute."""
+    import torch
+    nch = 64
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    plan.fdd_set_precision(dd.FDD_BF16)
+    plan.set_dm_list(np.linspace(0, 60, 600, dtype=np.float32))
+    nsamps = 1 << 20
+    md, ndm = plan.max_delay, plan.dm_count
+    rng = np.random.default_rng(5)
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    out = plan.execute(data, 8, 32)
+    d_in = torch.from_numpy(data).cuda()
+    d_out = torch.empty((ndm, nsamps - md), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(out, d_out.cpu().numpy())

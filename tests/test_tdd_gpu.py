@@ -287,3 +287,30 @@ def test_tdd_cfg4_beams_2bit(dd, orc, nsamps):
     for i in (358, 511, 1023):  # 358: nearest trial to the injected DM 700
         ref = orc.tdd(nch, delays, None, dms, nsamps, plan.max_delay, data, 2, 32, first_dm=i, dm_count=1)
         assert np.array_equal(d_out[i].cpu().numpy(), ref[0]), i
+
+
+def test_host_pointers_many_blocks_and_pieces(dd):
+    """Host-pointer execute with an output of several DM blocks (about 256 MB
+    each) and many 64 MiB transfer pieces, strided input and output rows
+    (execute_adv): identical to the device-pointer execute."""
+    import torch
+    nch, nbits = 64, 8
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.TDD)
+    plan.set_dm_list(np.linspace(0, 100, 150, dtype=np.float32))
+    md, ndm = plan.max_delay, plan.dm_count
+    nsamps = (1 << 20) + 12345
+    nsc = nsamps - md
+    rng = np.random.default_rng(77)
+    in_stride, out_stride = nch + 40, nsc * 4 + 256
+    host_in = np.zeros((nsamps, in_stride), np.uint8)
+    host_in[:, :nch] = rng.integers(0, 256, size=(nsamps, nch), dtype=np.uint8)
+    host_out = np.zeros((ndm, out_stride), np.uint8)
+    assert plan.execute_adv(nsamps, host_in, nbits, in_stride, host_out, 32, out_stride) == 0
+    d_in = torch.from_numpy(np.ascontiguousarray(host_in[:, :nch])).cuda()
+    d_out = torch.empty((ndm, nsc), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, nbits, d_out, 32, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    got = host_out[:, :nsc * 4].copy().view(np.float32)
+    assert np.array_equal(got, d_out.cpu().numpy())
+    assert not host_out[:, nsc * 4:].any()  # the row padding is not written
