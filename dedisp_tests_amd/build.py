@@ -72,8 +72,9 @@ def build(verbose: bool = True) -> Path:
     BIN.mkdir(exist_ok=True)
     for src in sorted(CLI.glob("*.cpp")):
         exe = BIN / src.stem
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{REPO / 'include'}", str(src), "-o",
-              str(exe) + ".tmp", f"-L{PKG}", "-ldedisp", "-Wl,-rpath,$ORIGIN/.."])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{REPO / 'include'}",
+              f"-I{ROCM / 'include'}", str(src), "-o", str(exe) + ".tmp", f"-L{PKG}", "-ldedisp",
+              f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM / 'lib'}"])
         os.replace(str(exe) + ".tmp", exe)
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", file=sys.stderr)

@@ -74,6 +74,8 @@ dedisp_error dedisp_create_plan(dedisp_plan* plan, dedisp_size nchans, dedisp_fl
     dedisp_error err = guarded([&] {
         switch (g_implementation) {
         case DEDISP_DEDISP:
+            p->ptr.reset(new dedisp::DedispPlan(nchans, dt, f0, df, g_device_idx));
+            break;
         case DEDISP_TDD:
             p->ptr.reset(new dedisp::TDDPlan(nchans, dt, f0, df, g_device_idx));
             break;

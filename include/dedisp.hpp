@@ -175,6 +175,17 @@ private:
 
 struct FDDState;
 
+/* The legacy plan (DEDISP_DEDISP, the C API's default selector). reference
+ * src/dedisp/DedispPlan.hpp:9-74, DedispPlan.cpp:24-50: same interface as
+ * TDDPlan (gulp size 65536, at most 8192 channels, execute / execute_adv /
+ * execute_guru) and the same per-sample integer arithmetic, so it runs the
+ * TDD kernels. */
+class DedispPlan : public TDDPlan {
+public:
+    DedispPlan(size_type nchans, float_type dt, float_type f0, float_type df, int device_idx = 0)
+        : TDDPlan(nchans, dt, f0, df, device_idx) {}
+};
+
 /* Fourier-domain dedispersion on the GPU. reference src/fdd/FDDGPUPlan.hpp,
  * src/fdd/FDDGPUPlan.cpp:39-706 (whole-series transform) and :725-1271
  * (time-segmented, overlap-save). Output formats and precision are chosen
