@@ -107,15 +107,18 @@ struct NoPrep {
 
 template <int A, int B, int C, int DIR, class Load, class Prep = NoPrep>
 __device__ __forceinline__ void fft3_core(const float2* __restrict__ ph, float* lds, Load load, float2 (&q)[A],
-                                          Prep prep = Prep(), int t1 = -1)
+                                          Prep prep = Prep(), int t1 = -1, int t3 = -1)
 {
     using S = FftShape<A, B, C>;
     constexpr int M = S::M;
     constexpr uint32_t MASK = 2 * M - 1;
     const int t = threadIdx.x;
     // pass-1 index of this thread (a permutation of 0..T1-1 over the threads
-    // t < T1 may be given; passes 2 and 3 use t itself)
+    // t < T1 may be given; pass 2 uses t itself) and pass-3 index (likewise a
+    // permutation of 0..T3-1 over t < T3: thread t then holds the output
+    // points k = t3 + T3*k0)
     const int tp = (t1 < 0) ? t : t1;
+    const int tq = (t3 < 0) ? t : t3;
 
     // pass 1: DFT over n2, then twiddle W_M^{tp k2} (recurrence reseeded every 8)
     float2 v[C];
@@ -174,7 +177,7 @@ __device__ __forceinline__ void fft3_core(const float2* __restrict__ ph, float* 
     }
 
     // exchange 2 -> 3: element (n0, k1, k2) at k2*(B*A + G2) + k1*A + n0
-    const int k2_3 = t % C, k1_3 = t / C;
+    const int k2_3 = tq % C, k1_3 = tq / C;
     const bool act3 = t < S::T3;
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
@@ -194,7 +197,7 @@ __device__ __forceinline__ void fft3_core(const float2* __restrict__ ph, float* 
             }
         }
     }
-    // pass 3: DFT over n0; output k = k2 + C k1 + C B k0 = t + T3 k0
+    // pass 3: DFT over n0; output k = k2 + C k1 + C B k0 = tq + T3 k0
     if (act3) dft_reg<A, DIR>(q);
 }
 
@@ -215,10 +218,14 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
 // real spectrum X[0..M] of the chunk (M+1 bins, row pitch Xp). The packed
 // transform Z of z[m] = x[2m] + i x[2m+1] is paired in the epilogue,
 //   X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]),
-// with Z[M-k] taken from the thread that holds it through LDS: point
-// k = t + T3*k0 has its mirror M-k at element A-1-k0 of thread T3-t (t > 0; for
-// t = 0 at element (A-k0) mod A of thread 0). Two rounds per component, each
-// moving half the elements, keep the extra registers to A/2 per component.
+// with Z[M-k] taken from the lane that holds it: pass 3 runs index tq on lane
+// i of a wave and T3 - tq on lane i + 32 (the inverse's mapping), so point
+// k = tq + T3*e has its mirror M-k at element A-1-e of the partner lane and
+// one v_permlane32_swap of that register serves both lanes (tq = T3/2: its
+// own element A-1-e; tq = 0: its own element (A-e) mod A). Measured as fast as
+// the earlier exchange of the mirrors through LDS (two rounds per component,
+// 3.73 vs 3.80 ms for 1024 ch x 23 chunks of 65536) with 92 instead of 127
+// registers.
 // TAB: the chunk's rows, first sample and valid length come from tab[chunk]
 // (the chunks of several beams in one launch) instead of rows/chunk_step/nvalid.
 template <int A, int B, int C, bool TAB = false, int NT = FftShape<A, B, C>::T>
@@ -230,8 +237,7 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
 {
     using S = FftShape<A, B, C>;
     static_assert(S::T3 == NT, "pass 3 uses every thread");
-    constexpr int M = S::M, T3 = S::T3, H = A / 2;
-    static_assert(S::LDS_FLOATS >= M, "a pairing round fits the LDS");
+    constexpr int M = S::M, T3 = S::T3;
     __shared__ float lds[S::LDS_FLOATS];
     const size_t chunk = blockIdx.x, c = blockIdx.y;
     size_t t0 = chunk * chunk_step;
@@ -275,43 +281,40 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
         }
         return make_float2(sample(t), sample(t + 1));
     };
-    float2 q[A];
-    fft3_core<A, B, C, -1>(ph, lds, load, q);
-
+    static_assert(T3 % 64 == 0, "mirror indices pair lanes i and i + 32 of whole waves");
+    // pass-3 index: index tq on lane i of a wave, its mirror T3 - tq on lane
+    // i + 32 (T3/2 is its own mirror, 0 pairs within its own elements)
     const int t = threadIdx.x;
+    const int hi = (t >> 5) & 1;
+    const int base = (t >> 6) * 32 + (t & 31);
+    const int tq = hi ? (base ? T3 - base : T3 / 2) : base;
+    const bool self0 = (tq == 0), selfh = (tq == T3 / 2);
+    float2 q[A];
+    fft3_core<A, B, C, -1>(ph, lds, load, q, NoPrep(), -1, tq);
+    // the value of the lane i ^ 32
+    auto other_half = [&](float x) -> float {
+        const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
+                                                        __builtin_bit_cast(uint32_t, x), false, false);
+        return __builtin_bit_cast(float, hi ? r[0] : r[1]);
+    };
+    // point k = tq + T3*e has its mirror M - k at element A-1-e of the partner
+    // lane (one swap of that register serves both lanes)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        // round h: elements [H(1-h), H(2-h)) go to LDS, elements [hH, hH+H)
-        // read their mirrors (slot (H-1-j)*T3 + T3-t for element hH+j)
-        const int wbase = h ? 0 : H, rbase = h ? H : 0;
-        float mx[H], my[H];
-#pragma unroll
-        for (int comp = 0; comp < 2; ++comp) {
-            __syncthreads();
-#pragma unroll
-            for (int e = 0; e < H; ++e) lds[e * T3 + t] = comp ? q[wbase + e].y : q[wbase + e].x;
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < H; ++j) {
-                const float v = lds[(H - 1 - j) * T3 + (T3 - t)];  // in bounds: < 2*H*T3 = M
-                const float own = comp ? q[rbase].y : q[rbase].x;  // t = 0, j = 0: own mirror
-                if (comp) my[j] = (t == 0 && j == 0) ? own : v;
-                else mx[j] = (t == 0 && j == 0) ? own : v;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const int k = t + T3 * (rbase + j);
-            const float2 a = q[rbase + j];
-            const float2 pk = ph[k];  // W^k = conj(pk)
-            const float ux = a.x + mx[j], uy = a.y - my[j];  // Z[k] + conj Z[M-k]
-            const float vx = a.x - mx[j], vy = a.y + my[j];  // Z[k] - conj Z[M-k]
-            // -i/2 W^k v = 1/2 (Im(W^k v), -Re(W^k v))
-            const float wr = vx * pk.x + vy * pk.y, wi = vy * pk.x - vx * pk.y;
-            dst[k] = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
-        }
+    for (int e = 0; e < A; ++e) {
+        const float2 own = q[A - 1 - e];
+        float2 mz = make_float2(other_half(own.x), other_half(own.y));
+        if (selfh) mz = own;
+        if (self0) mz = q[(A - e) % A];
+        const int k = tq + T3 * e;
+        const float2 a = q[e];
+        const float2 pk = ph[k];  // W^k = conj(pk)
+        const float ux = a.x + mz.x, uy = a.y - mz.y;  // Z[k] + conj Z[M-k]
+        const float vx = a.x - mz.x, vy = a.y + mz.y;  // Z[k] - conj Z[M-k]
+        // -i/2 W^k v = 1/2 (Im(W^k v), -Re(W^k v))
+        const float wr = vx * pk.x + vy * pk.y, wi = vy * pk.x - vx * pk.y;
+        dst[k] = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
     }
-    if (t == 0) dst[M] = make_float2(q[0].x - q[0].y, 0.f);  // X[M] = Re Z[0] - Im Z[0]
+    if (self0) dst[M] = make_float2(q[0].x - q[0].y, 0.f);  // X[M] = Re Z[0] - Im Z[0]
 }
 
 // Inverse: DM row j, chunk i of Y (M+1 bins); writes the first `good`
