@@ -148,7 +148,7 @@ def cpu_baseline(seconds_budget: float = 30.0):
 MFMA_PEAK_T = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # dense TFLOP/s
 
 
-def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16")) -> dict:
+def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), e2e: bool = True) -> dict:
     """FDD on 1024 ch x 2^20 8-bit samples (N(128,20), five pulses), 2048 linear DM
     trials 0..1023.5, overlap-save chunks of 2^16 samples, time-series output,
     input and output in HBM. One line per contraction form with its error against
@@ -223,7 +223,7 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16")) 
                          "frac": tflops / MFMA_PEAK_T[name],
                          "note": "algorithmic flops (8 per complex MAC, no padding or compensation products)"},
         }
-        if name == "bf16":
+        if name == "bf16" and e2e:
             # end-to-end scope: the same execute with the caller's pageable host buffers
             h_out = np.zeros((2048, nsc), np.float32)
             assert plan.execute_raw(NSAMPS, host_in, 8, h_out, 32, 0) == 0
@@ -468,6 +468,11 @@ def e2e_leg(dd, plan, host_in, d_out, ndm: int, nsc: int, steps: int = 3) -> dic
             "note": "host pointers (pageable numpy buffers): H2D + transpose + channel sum + D2H"}
 
 
+def progress(leg: str, t_start: float) -> None:
+    """One line on stderr when a leg has finished (stdout carries only the JSON line)."""
+    print(f"bench: {leg} leg done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
+
 def valu_roofline(k_ms: float, adds_t: float) -> dict:
     """VALU issue rate of the channel-sum kernel: the measured instruction count
     per launch (committed counter summary) over the kernel time, against the
@@ -494,6 +499,8 @@ def main():
     ap.add_argument("--no-fdd", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the host-pointer runs (profiling: they launch the same kernels on DM blocks)")
     ap.add_argument("--fdd-forms", default="fp32,bf16,fp16",
                     help="contraction forms of the FDD leg (counter runs use one)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -517,6 +524,7 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes": x.numel() * 4, "copies": 2}))
         return
 
+    t_start = time.perf_counter()
     s, d_in = make_input(torch)
     plan = dd.Plan(NCH, DT, F0, DF, impl=dd.TDD)
     plan.generate_dm_list(0.0, 1000.0, 40.0, 1.25)
@@ -587,21 +595,28 @@ def main():
         "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
         "parity": parity,
     }
-    line["end_to_end"] = e2e_leg(dd, plan, d_in.cpu().numpy(), d_out, ndm, nsc)
+    progress("tdd", t_start)
+    if not args.no_e2e:
+        line["end_to_end"] = e2e_leg(dd, plan, d_in.cpu().numpy(), d_out, ndm, nsc)
+        progress("end_to_end", t_start)
     del d_out, d_in, s
     torch.cuda.empty_cache()
     if not args.no_fdd:
         line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1 if args.warmup else 0,
-                              forms=args.fdd_forms.split(","))
+                              forms=args.fdd_forms.split(","), e2e=not args.no_e2e)
         torch.cuda.empty_cache()
+        progress("fdd", t_start)
     if not args.no_batch:
         line["batch"] = batch_leg(torch, dd)
         torch.cuda.empty_cache()
+        progress("batch", t_start)
     if not args.no_stream:
         line["stream"] = stream_leg(torch, dd)
         torch.cuda.empty_cache()
+        progress("stream", t_start)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
+        progress("cpu_baseline", t_start)
     print(json.dumps(line))
 
 
