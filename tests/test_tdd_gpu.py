@@ -314,3 +314,26 @@ def test_host_pointers_many_blocks_and_pieces(dd):
     got = host_out[:, :nsc * 4].copy().view(np.float32)
     assert np.array_equal(got, d_out.cpu().numpy())
     assert not host_out[:, nsc * 4:].any()  # the row padding is not written
+
+
+def test_host_pointers_rows_longer_than_a_transfer_slot(dd):
+    """Output rows of 18 M samples (72 MB, longer than one 64 MiB page-locked
+    slot, so each row moves in segments), 1-bit input of 64 channels: the
+    host-pointer execute equals the device-pointer one."""
+    import torch
+    nch, nbits = 64, 1
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.TDD)
+    plan.set_dm_list(np.array([0.0, 7.0, 30.0], np.float32))
+    md, ndm = plan.max_delay, plan.dm_count
+    nsamps = 18_000_000
+    nsc = nsamps - md
+    assert nsc * 4 > (64 << 20)
+    rng = np.random.default_rng(3)
+    host_in = rng.integers(0, 256, size=(nsamps, nch // 8), dtype=np.uint8)
+    out = plan.execute(host_in, nbits, 32)
+    d_in = torch.from_numpy(host_in).cuda()
+    d_out = torch.empty((ndm, nsc), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, nbits, d_out, 32, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(out, d_out.cpu().numpy())
