@@ -127,7 +127,7 @@ int main(int argc, char** argv)
     auto t0 = std::chrono::steady_clock::now();
     e = dedisp_amd_fil_read_data(filename, &h, input.data(), input.size());
     if (e != DEDISP_NO_ERROR) return fail("reading the samples", e);
-    if (verbose) std::printf("Reading input file took %.2f seconds\n", seconds_since(t0));
+    if (verbose) std::printf("Reading input file took %.3f seconds\n", seconds_since(t0));
     size_t nsamps = (size_t)h.nsamps;
     if (ndec > 1) {
         nsamps = dedisp_amd_fil_decimate(input.data(), (size_t)h.nchans, nsamps, ndec);
@@ -171,7 +171,7 @@ int main(int argc, char** argv)
     e = dedisp_execute(plan, nsamps, input.data(), (size_t)h.nbits, reinterpret_cast<dedisp_byte*>(output.data()),
                        32, DEDISP_USE_DEFAULT);
     if (e != DEDISP_NO_ERROR) return fail("dedispersing", e);
-    if (verbose) std::printf("Dedispersion took %.2f seconds\n", seconds_since(t0));
+    if (verbose) std::printf("Dedispersion took %.3f seconds\n", seconds_since(t0));
 
     if (numout == 0 || numout > nsc) numout = nsc;
     numout &= ~size_t(1);  // even length (fil.hpp:262-266)
@@ -182,7 +182,7 @@ int main(int argc, char** argv)
         if ((e = dedisp_amd_write_inf(prefix.c_str(), &h, dms[i], numout)) != DEDISP_NO_ERROR)
             return fail("writing an .inf file", e);
     }
-    if (verbose) std::printf("Writing DM trials took %.2f seconds\n", seconds_since(t0));
+    if (verbose) std::printf("Writing DM trials took %.3f seconds\n", seconds_since(t0));
     dedisp_destroy_plan(plan);
     return 0;
 }
