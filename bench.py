@@ -182,8 +182,14 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
     dms = (np.arange(2048, dtype=np.float32) * np.float32(0.5)).astype(np.float32)
     host_in = d_in.cpu().numpy()
     sample_rows = [0, 700, 1400, 2047]
-    ref = oracle.fdd_timeseries(NCH, oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF)),
-                                dms[sample_rows], NSAMPS, 0, host_in)
+    otab = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
+    ref = oracle.fdd_timeseries(NCH, otab, dms[sample_rows], NSAMPS, 0, host_in)
+    # the DM trials nearest the injected pulses, for the S/N comparison
+    pulse_rows = [int(round(dm / 0.5)) for dm in pulse_dms]
+    ref_p = oracle.fdd_timeseries(NCH, otab, dms[pulse_rows], NSAMPS, 0, host_in)
+
+    def snr(x):
+        return (x.max() - np.median(x)) / x.std()
     out = {"workload": "fdd-1024ch-2^20x8bit-2048dm-nfft65536", "ndm": 2048, "nfft": 65536}
     for name, prec in (("fp32", dd.FDD_FP32), ("bf16", dd.FDD_BF16), ("fp16", dd.FDD_FP16)):
         if name not in forms:
@@ -212,6 +218,8 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
         r = ref[:, :nsc]
         rel = float(np.linalg.norm(got - r) / np.linalg.norm(r))
         maxabs = float(np.max(np.abs(got - r)))
+        got_p = d_out[pulse_rows].cpu().numpy().astype(np.float64)
+        snr_err = max(abs(snr(got_p[j]) / snr(ref_p[j, :nsc]) - 1.0) for j in range(len(pulse_rows)))
         c_ms = float(np.mean([p["dedisperse_ms"] for p in prof]))
         cmacs = prof[-1]["work_units"]
         tflops = 8 * cmacs / (c_ms * 1e-3) / 1e12
@@ -222,6 +230,9 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
                           "contraction": c_ms,
                           "inverse_fft": float(np.mean([p["ifft_ms"] for p in prof]))},
             "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": maxabs,
+            "pulse_snr_rel_err": float(snr_err),
+            "pulse_snr_note": f"max over the 5 injected pulses of |S/N / S/N(FP64) - 1| on the nearest DM trial "
+                              f"(rows {pulse_rows}); S/N = (max - median) / std of the row",
             "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_PEAK_T[name], "unit": "TFLOP/s",
                          "frac": tflops / MFMA_PEAK_T[name],
                          "note": "algorithmic flops (8 per complex MAC, no padding or compensation products)"},
