@@ -30,6 +30,8 @@ Prints one JSON line. Besides the contract fields it carries:
                  calling convention): PCIe-inclusive rate and stage times
   filterbank     the dedisp_fil tool on the headline input written as a
                  SIGPROC file: read, dedisperse 64 trials, write .dat/.inf
+  measured_peaks achievable HBM copy bandwidth and bf16 / fp32 GEMM rates on
+                 this GPU; the rooflines also give fractions of these
   cpu_baseline   the reference's CPU FDD plan (restated as written, OpenMP)
                  timed on this host on a bounded DM sample, scaled to the
                  full DM list
@@ -536,6 +538,41 @@ def e2e_leg(dd, plan, host_in, d_out, ndm: int, nsc: int, steps: int = 3) -> dic
             "note": "host pointers (pageable numpy buffers): H2D + transpose + channel sum + D2H"}
 
 
+def peaks_leg(torch) -> dict:
+    """Achievable peaks on this GPU, beside the nominal ones: HBM by a 1 GiB
+    device-to-device copy (read + write bytes), the matrix cores by 8192^3
+    bf16, fp16 and fp32 GEMMs (torch.matmul on the vendor library; 2 n^3 flops).
+    Timed with events on torch's current stream, median of 10."""
+    def timed(f, reps=10):
+        f()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            f()
+            b.record()
+            b.synchronize()
+            ts.append(a.elapsed_time(b) * 1e-3)
+        return float(np.median(ts))
+    n_copy = 1 << 28  # floats = 1 GiB
+    x = torch.empty(n_copy, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    t = timed(lambda: y.copy_(x))
+    out = {"hbm_copy_gbs": 2 * 4 * n_copy / t / 1e9}
+    del x, y
+    n = 8192
+    for name, dt in (("bf16", torch.bfloat16), ("fp16", torch.float16), ("fp32", torch.float32)):
+        a = torch.randn((n, n), device="cuda").to(dt)
+        b = torch.randn((n, n), device="cuda").to(dt)
+        t = timed(lambda: torch.matmul(a, b), reps=5)
+        out[f"{name}_gemm_tflops"] = 2.0 * n ** 3 / t / 1e12
+        del a, b
+    torch.cuda.empty_cache()
+    out["note"] = "nominal: HBM 8000 GB/s, bf16 / fp16 2500 and fp32 157.3 TFLOP/s dense"
+    return out
+
+
 def progress(leg: str, t_start: float) -> None:
     """One line on stderr when a leg has finished (stdout carries only the JSON line)."""
     print(f"bench: {leg} leg done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
@@ -664,6 +701,10 @@ def main():
         "parity": parity,
     }
     progress("tdd", t_start)
+    peaks = peaks_leg(torch)
+    line["measured_peaks"] = peaks
+    line["roofline"]["frac_of_measured_peak"] = achieved_gbs / peaks["hbm_copy_gbs"]
+    progress("peaks", t_start)
     if not args.no_e2e:
         host_in = d_in.cpu().numpy()
         line["end_to_end"] = e2e_leg(dd, plan, host_in, d_out, ndm, nsc)
@@ -676,6 +717,10 @@ def main():
     if not args.no_fdd:
         line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1 if args.warmup else 0,
                               forms=args.fdd_forms.split(","), e2e=not args.no_e2e)
+        for form, key in (("fp32", "fp32_gemm_tflops"), ("bf16", "bf16_gemm_tflops"), ("fp16", "fp16_gemm_tflops")):
+            if form in line["fdd"]:
+                rl = line["fdd"][form]["roofline"]
+                rl["frac_of_measured_gemm"] = rl["achieved"] / peaks[key]
         torch.cuda.empty_cache()
         progress("fdd", t_start)
     if not args.no_batch:
