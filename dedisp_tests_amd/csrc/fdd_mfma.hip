@@ -10,11 +10,15 @@
 // Phasors (A operand), generated in registers:
 //   phase in units of 2 pi/65536: q = (d * 65536/nfft) mod 65536 per (DM,
 //   channel), precomputed as uint16; the phase of bin k is k*q mod 65536
-//   (exact integer arithmetic). e^{2 pi i m/65536} = T_hi[m>>8] * T_lo[m&255]
-//   from two 256-entry LDS tables built in double precision. Per (DM,
-//   channel) and workgroup two lookups, P at the first bin and the step
-//   W = e^{2 pi i q/65536}; the following bins are P <- P*W (NB-1 complex
-//   products, relative error ~1e-7 after NB-1 = 3 steps).
+//   (exact integer arithmetic). Per (DM, channel) and workgroup P at the first
+//   bin and the step W = e^{2 pi i q/65536} come from the hardware cosine and
+//   sine (v_cos_f32 / v_sin_f32 take the angle in revolutions, and m/65536 is
+//   exact in float); the following bins are P <- P*W (NB-1 complex products).
+//   With DD_MFMA_TRANS=0 P and W are instead T_hi[m>>8] * T_lo[m&255] from two
+//   256-entry LDS tables built in double (relative error ~1e-7): the same
+//   accuracy at the oracle (FP32 rel-L2 6.5e-7 vs 6.7e-7) but slower (bf16
+//   59.9 vs 56.9 ms, fp16 36.6 vs 34.4 ms), the random-address table reads
+//   costing LDS time and bank conflicts.
 // Spectra (B operand): X_k[c][chunk] for the K-step, staged in LDS in the
 //   planes of the form (float32: re, im, -im; bfloat16: re hi, re lo, im hi,
 //   im lo and the negated im planes; float16: re, im, -im), formed on load from
@@ -56,6 +60,9 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 #endif
 #ifndef DD_MFMA_STAGGER
 #define DD_MFMA_STAGGER 1
+#endif
+#ifndef DD_MFMA_TRANS
+#define DD_MFMA_TRANS 1
 #endif
 
 #ifndef DD_MFMA_WAVES
@@ -166,9 +173,11 @@ void fdd_mfma_k(
     const uint32_t k0 = bblk * NB;
     const uint32_t ch0 = cgrp * NJ;
 
-    for (int t = tid; t < 512; t += NT)
-        reinterpret_cast<float2*>(tab)[t] = tab65536[t];
-    __syncthreads();  // the table is read before the first K-step barrier
+    if (!DD_MFMA_TRANS) {
+        for (int t = tid; t < 512; t += NT)
+            reinterpret_cast<float2*>(tab)[t] = tab65536[t];
+        __syncthreads();  // the table is read before the first K-step barrier
+    }
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
     // Every load is unconditional and in bounds (X has its channel count
@@ -252,11 +261,12 @@ void fdd_mfma_k(
         for (int r = 0; r < 16; ++r) acc_r[b][r] = acc_i[b][r] = 0.f;
 
     // Phasors of bin k0 and the per-bin steps for this lane's 8 channels. The
-    // LDS table reads of a K-step are issued before any product uses them.
-    // The second half of the waves computes them for the next K-step at the
-    // end of the current one (the registers are free by then), so after each
-    // barrier one wave of a SIMD issues MFMAs while the other waits on LDS.
-    // With DD_MFMA_LOPOLY the bfloat16 and float16 forms evaluate the low
+    // second half of the waves computes them for the next K-step at the end
+    // of the current one (the registers are free by then), so after each
+    // barrier one wave of a SIMD issues MFMAs while the other generates
+    // phasors. With the LDS tables (DD_MFMA_TRANS=0) a K-step's table reads
+    // are issued before any product uses them, and with DD_MFMA_LOPOLY the
+    // bfloat16 and float16 forms evaluate the low
     // factor e^{2 pi i j/65536} (j < 256, angle below 0.025) as
     // (1 - t^2/2 + t^4/24, t - t^3/6), truncation error below 1e-10, instead
     // of reading T_lo. That halves the random-address LDS reads but measured
@@ -272,6 +282,20 @@ void fdd_mfma_k(
                            t * __builtin_fmaf(t2, -1.f / 6.f, 1.f));
     };
     float2 p[8], w[8];
+#if DD_MFMA_TRANS
+    // v_cos_f32 / v_sin_f32 take the angle in revolutions: (k0*q mod 65536)/65536
+    // and q/65536 are exact in float
+    auto lookups = [&](const u32x4& qq) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t q = (qq[s >> 1] >> (16 * (s & 1))) & 0xffffu;
+            const float am = (float)(__umul24(k0, q) & 0xffffu) * (1.f / 65536.f);
+            const float aw = (float)q * (1.f / 65536.f);
+            p[s] = make_float2(__builtin_amdgcn_cosf(am), __builtin_amdgcn_sinf(am));
+            w[s] = make_float2(__builtin_amdgcn_cosf(aw), __builtin_amdgcn_sinf(aw));
+        }
+    };
+#else
     auto lookups = [&](const u32x4& qq) {
         float2 th[8], tl[8], wh[8], wl[8];
 #pragma unroll
@@ -294,6 +318,7 @@ void fdd_mfma_k(
             w[s] = cmul(wh[s], wl[s]);
         }
     };
+#endif
     const bool late = DD_MFMA_STAGGER && WAVES == 8 && wave >= WAVES / 2;
 
     const uint32_t nks = (nchan + KS - 1) / KS;
