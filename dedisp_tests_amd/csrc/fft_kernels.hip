@@ -408,12 +408,15 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     float2 q[A];
     fft3_core<A, B, C, +1>(ph, lds, load, q, pair, tp);
     if (t < S::T3) {
+        // one workgroup-uniform base and limit (t0 < nsamps_out for every
+        // chunk processed); per sample a 32-bit offset
+        float* const base = dst + t0;
+        const uint32_t lim = (uint32_t)min(good, nsamps_out - t0);
 #pragma unroll
         for (int k0 = 0; k0 < A; ++k0) {
-            const size_t m = (size_t)t + (size_t)S::T3 * k0;
-            const size_t ta = 2 * m, tb = ta + 1;
-            if (ta < good && t0 + ta < nsamps_out) dst[t0 + ta] = q[k0].x * inv_n;
-            if (tb < good && t0 + tb < nsamps_out) dst[t0 + tb] = q[k0].y * inv_n;
+            const uint32_t ta = 2u * ((uint32_t)t + (uint32_t)S::T3 * k0);
+            if (ta < lim) base[ta] = q[k0].x * inv_n;
+            if (ta + 1 < lim) base[ta + 1] = q[k0].y * inv_n;
         }
     }
 }
