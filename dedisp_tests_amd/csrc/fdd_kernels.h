@@ -89,7 +89,8 @@ hipError_t dd_fdd_contract_fp32(const float2* X, size_t Xp, const int32_t* delay
 // accumulation in all three. Same output as dd_fdd_contract_fp32; the delays
 // enter as phase steps q16[i][c] = (d[i][c] * 65536/nfft) mod 65536 (row pitch
 // qpitch, a multiple of 16 and >= nchan rounded up to 16, zero padded), and
-// tab65536 holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
+// tab65536 (read only by the kernel's table form, DD_MFMA_TRANS=0; the default
+// takes the hardware cosine and sine) holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
 // (b < 256). The spectra are multiplied by xscale before the MFMA inputs are
 // formed and Y by 1/xscale after (a power of two keeping float16 in range).
 // X must hold nchan rounded up to 16 channel rows, the extra rows zero.

@@ -134,6 +134,7 @@ __device__ __forceinline__ uint32_t pack_f16x2(float a, float b)
     return __builtin_bit_cast(uint32_t, v);
 }
 
+// Table form (DD_MFMA_TRANS=0):
 // e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used) is T_hi[m>>8] * T_lo[m&255];
 // the LDS byte offsets of the two entries are (m8 >> 8) & 0x7f8 and
 // 2048 + (m8 & 0x7f8).
@@ -154,7 +155,9 @@ void fdd_mfma_k(
     using F = Form<PREC>;
     constexpr int BUF = F::PLANES * SLAB * F::ELEM;  // bytes of one staged K-step
     __shared__ __attribute__((aligned(16))) char xs[2 * BUF];
+#if !DD_MFMA_TRANS
     __shared__ __attribute__((aligned(16))) char tab[4096];  // T_hi[256], T_lo[256]
+#endif
 
     const uint32_t M = nfft / 2;
     const int tid = threadIdx.x;
@@ -173,11 +176,11 @@ void fdd_mfma_k(
     const uint32_t k0 = bblk * NB;
     const uint32_t ch0 = cgrp * NJ;
 
-    if (!DD_MFMA_TRANS) {
-        for (int t = tid; t < 512; t += NT)
-            reinterpret_cast<float2*>(tab)[t] = tab65536[t];
-        __syncthreads();  // the table is read before the first K-step barrier
-    }
+#if !DD_MFMA_TRANS
+    for (int t = tid; t < 512; t += NT)
+        reinterpret_cast<float2*>(tab)[t] = tab65536[t];
+    __syncthreads();  // the table is read before the first K-step barrier
+#endif
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
     // Every load is unconditional and in bounds (X has its channel count

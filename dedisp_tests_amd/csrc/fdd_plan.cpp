@@ -266,7 +266,8 @@ size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
 
 // Phase tables of segment length nfft: e^{2 pi i m/nfft} for the transforms
 // and, for the matrix-core contraction (valu false), the 65536-based phase
-// steps and the two-level phasor tables. Returns the float16 spectrum scale.
+// steps (and the two-level phasor tables its table form reads). Returns the
+// float16 spectrum scale.
 float FDDGPUPlan::prepare_phasors(size_t nfft, bool valu)
 {
     FDDState& st = *m_state;
