@@ -113,6 +113,21 @@ FDDGPUPlan::FDDGPUPlan(size_type nchans, float_type dt, float_type f0, float_typ
 
 FDDGPUPlan::~FDDGPUPlan() {}
 
+// The CPU FDD algorithm is not part of this GPU library (src/fdd/FDDCPUPlan.cpp
+// is restated only as a test oracle): constructing the plan fails loudly.
+FDDCPUPlan::FDDCPUPlan(size_type nchans, float_type dt, float_type f0, float_type df, int)
+    : Plan(nchans, dt, f0, df)
+{
+    DD_THROW(DEDISP_UNKNOWN_ERROR);
+}
+
+FDDCPUPlan::~FDDCPUPlan() {}
+
+void FDDCPUPlan::execute(size_type, const byte_type*, size_type, byte_type*, size_type, unsigned)
+{
+    DD_THROW(DEDISP_UNKNOWN_ERROR);
+}
+
 void FDDGPUPlan::set_output_mode(dedisp_amd_fdd_output mode)
 {
     if (mode != DEDISP_AMD_FDD_FOURIER && mode != DEDISP_AMD_FDD_TIMESERIES) DD_THROW(DEDISP_UNKNOWN_ERROR);

@@ -240,6 +240,20 @@ private:
     std::unique_ptr<FDDState> m_state;
 };
 
+/* The reference's CPU FDD plan (src/fdd/FDDCPUPlan.hpp:14-40), declared so
+ * code written against the reference's C++ API compiles and links. This
+ * library computes every plan on the GPU and does not provide the CPU
+ * algorithm: constructing one throws dedisp::Error with
+ * DEDISP_UNKNOWN_ERROR, as dedisp_create_plan refuses USE_CPU=1 for FDD. */
+class FDDCPUPlan : public Plan {
+public:
+    FDDCPUPlan(size_type nchans, float_type dt, float_type f0, float_type df, int device_index = 0);
+    ~FDDCPUPlan();
+    virtual void execute(size_type nsamps, const byte_type* in, size_type in_nbits,
+                         byte_type* out, size_type out_nbits,
+                         unsigned flags = 0) override;
+};
+
 } // namespace dedisp
 
 #endif // DEDISP_HPP_INCLUDE_GUARD
