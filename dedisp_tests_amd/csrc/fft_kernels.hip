@@ -377,7 +377,9 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
         const float2 o = cmul(d, n2 ? cmul(ptp, w) : ptp);
         return make_float2(e.x - o.y, e.y + o.x);
     };
-    auto load = [&](int k) -> float2 { return src[k]; };
+    // a 32-bit offset from the workgroup-uniform row base (the scalar-base
+    // load form: one offset register per load instead of a 64-bit address)
+    auto load = [&](int k) -> float2 { return src[(uint32_t)k]; };
     auto pair = [&](float2 (&v)[C], bool act) {
         if (!act) return;
         if (self0) {
