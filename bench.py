@@ -223,14 +223,25 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
         got_p = d_out[pulse_rows].cpu().numpy().astype(np.float64)
         snr_err = max(abs(snr(got_p[j]) / snr(ref_p[j, :nsc]) - 1.0) for j in range(len(pulse_rows)))
         c_ms = float(np.mean([p["dedisperse_ms"] for p in prof]))
+        st_ms = {"transpose": float(np.mean([p["unpack_ms"] for p in prof])),
+                 "forward_fft": float(np.mean([p["fft_ms"] for p in prof])),
+                 "contraction": c_ms,
+                 "inverse_fft": float(np.mean([p["ifft_ms"] for p in prof]))}
+        # algorithmic HBM bytes of the transform stages: packed input once, the
+        # spectra X[c][chunk][bin] and Y[dm][chunk][bin] (complex float), the
+        # float output rows
+        nchunk = -(-nsc // (65536 - maxd))
+        x_bytes = NCH * nchunk * (65536 // 2 + 1) * 8
+        y_bytes = 2048 * nchunk * (65536 // 2 + 1) * 8
+        st_bytes = {"transpose": 2 * NSAMPS * NCH, "forward_fft": NSAMPS * NCH + x_bytes,
+                    "inverse_fft": y_bytes + 2048 * nsc * 4}
         cmacs = prof[-1]["work_units"]
         tflops = 8 * cmacs / (c_ms * 1e-3) / 1e12
         out[name] = {
             "value": 2048 * nsc / el, "ms_per_step": 1e3 * el,
-            "stages_ms": {"transpose": float(np.mean([p["unpack_ms"] for p in prof])),
-                          "forward_fft": float(np.mean([p["fft_ms"] for p in prof])),
-                          "contraction": c_ms,
-                          "inverse_fft": float(np.mean([p["ifft_ms"] for p in prof]))},
+            "stages_ms": st_ms,
+            "stages_hbm": {k: {"algorithmic_bytes": v, "gbs": v / (st_ms[k] * 1e-3) / 1e9}
+                           for k, v in st_bytes.items()},
             "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": maxabs,
             "pulse_snr_rel_err": float(snr_err),
             "pulse_snr_note": f"max over the 5 injected pulses of |S/N / S/N(FP64) - 1| on the nearest DM trial "
@@ -569,8 +580,26 @@ def peaks_leg(torch) -> dict:
         out[f"{name}_gemm_tflops"] = 2.0 * n ** 3 / t / 1e12
         del a, b
     torch.cuda.empty_cache()
-    out["note"] = "nominal: HBM 8000 GB/s, bf16 / fp16 2500 and fp32 157.3 TFLOP/s dense"
+    out.update(int_add_peak())
+    out["note"] = ("nominal: HBM 8000 GB/s, bf16 / fp16 2500 and fp32 157.3 TFLOP/s dense, "
+                   "32-bit integer adds 256 CU x 4 SIMD x 64 lanes / 2 cycles x 2.4 GHz = 7.86e13/s")
     return out
+
+
+def int_add_peak() -> dict:
+    """Achievable 32-bit integer add rate: libdd_peaks.so's probe (8 independent
+    v_add_u32 chains per thread, 8 waves on every SIMD), fastest of 5 runs."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "dedisp_tests_amd", "libdd_peaks.so"))
+    f = lib.dd_peak_int_add
+    f.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    f.restype = ctypes.c_int
+    adds, insts = ctypes.c_double(), ctypes.c_double()
+    rc = f(65536, 4096, 5, ctypes.byref(adds), ctypes.byref(insts))
+    if rc != 0:
+        raise RuntimeError(f"dd_peak_int_add failed: HIP error {rc}")
+    return {"int_add_lane_adds_per_s": adds.value, "valu_wave_insts_per_s": insts.value}
 
 
 def progress(leg: str, t_start: float) -> None:
@@ -704,6 +733,10 @@ def main():
     peaks = peaks_leg(torch)
     line["measured_peaks"] = peaks
     line["roofline"]["frac_of_measured_peak"] = achieved_gbs / peaks["hbm_copy_gbs"]
+    vr = line["valu_roofline"]
+    vr["adds_frac_of_measured_int_add"] = vr["adds_per_s"] / peaks["int_add_lane_adds_per_s"]
+    if vr["achieved"] is not None:
+        vr["frac_of_measured_issue"] = vr["achieved"] / peaks["valu_wave_insts_per_s"]
     progress("peaks", t_start)
     if not args.no_e2e:
         host_in = d_in.cpu().numpy()
@@ -721,6 +754,8 @@ def main():
             if form in line["fdd"]:
                 rl = line["fdd"][form]["roofline"]
                 rl["frac_of_measured_gemm"] = rl["achieved"] / peaks[key]
+                for st in line["fdd"][form]["stages_hbm"].values():
+                    st["frac_of_measured_hbm"] = st["gbs"] / peaks["hbm_copy_gbs"]
         torch.cuda.empty_cache()
         progress("fdd", t_start)
     if not args.no_batch:

@@ -1,4 +1,5 @@
-"""Builds dedisp_tests_amd/libdedisp.so in-tree for gfx950.
+"""Builds dedisp_tests_amd/libdedisp.so in-tree for gfx950 (and the bench's
+peak-rate probes, libdd_peaks.so).
 
 Host sources (*.cpp) are compiled with g++ against the HIP runtime headers;
 kernel sources (*.hip) with hipcc --offload-arch=gfx950. Objects go to
@@ -20,6 +21,8 @@ REPO = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libdedisp.so"
+PEAKS = PKG / "peaks"
+PEAKS_LIB = PKG / "libdd_peaks.so"
 CLI = PKG / "cli"
 BIN = PKG / "bin"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
@@ -68,6 +71,14 @@ def build(verbose: bool = True) -> Path:
           *map(str, objs), "-o", str(tmp), f"-L{ROCM / 'lib'}", "-lamdhip64",
           f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
+    # the bench's peak-rate probes: a library of their own, not linked into libdedisp.so
+    psrc = sorted(PEAKS.glob("*.hip"))
+    pobjs = [_compile(src) for src in psrc]
+    ptmp = PEAKS_LIB.with_suffix(".so.tmp")
+    _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC",
+          *map(str, pobjs), "-o", str(ptmp), f"-L{ROCM / 'lib'}", "-lamdhip64",
+          f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(ptmp, PEAKS_LIB)
     # command-line front ends (host only, linked against the library)
     BIN.mkdir(exist_ok=True)
     for src in sorted(CLI.glob("*.cpp")):

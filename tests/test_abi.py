@@ -48,6 +48,17 @@ def test_library_has_gfx950_code(dd):
     assert b"gfx950" in data
 
 
+def test_peak_probe_library_is_separate(dd):
+    # the bench's integer-add probe ships in its own library, not in libdedisp.so
+    peaks = dd.LIB_PATH.parent / "libdd_peaks.so"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(peaks)], capture_output=True,
+                         text=True, check=True).stdout
+    assert "dd_peak_int_add" in out
+    assert b"gfx950" in peaks.read_bytes()
+    syms = subprocess.run(["nm", "-D", str(dd.LIB_PATH)], capture_output=True, text=True).stdout
+    assert "dd_peak_" not in syms
+
+
 def test_error_strings(dd):
     # src/common/dedisp_error.cpp:3-42
     assert dd.error_string(dd.DEDISP_NO_ERROR) == "No error"
