@@ -131,6 +131,108 @@ __global__ __launch_bounds__(256) void tdd_transpose_k(
     }
 }
 
+// Transpose for 2-, 4- and 8-bit samples, where one 32-bit word of a sample
+// row holds SPW = 32/NBITS channels and one output word SPW samples: a thread
+// reads one word from each of SPW consecutive sample rows (an SPW x SPW matrix
+// of NBITS-bit elements, row = sample, column = channel) and transposes it in
+// registers, log2(SPW) stages that swap the off-diagonal blocks of each
+// pair of rows (shift + bitfield insert per word), giving SPW output words.
+// The tile is stored 32 words per sample row with word d of row t at
+// d ^ ((t / SPW) & 31): the 32 lanes that read word d of rows SPW*wl + r for
+// wl = 0..31 hit 32 different banks, and a row's stores stay a permutation.
+
+constexpr uint32_t low_blocks_mask(int sh)
+{
+    // the low sh bits of every 2*sh-bit group
+    uint32_t m = 0;
+    for (int b = 0; b < 32; ++b)
+        if ((b / sh) % 2 == 0) m |= 1u << b;
+    return m;
+}
+
+template <int NBITS>
+__global__ __launch_bounds__(256) void tdd_transpose_reg_k(
+    const uint8_t* __restrict__ in, size_t nsamps, size_t in_stride, size_t row_bytes,
+    size_t nchans, uint32_t* __restrict__ out, size_t row_words)
+{
+    static_assert(NBITS == 2 || NBITS == 4 || NBITS == 8, "register transpose: 2-, 4- or 8-bit");
+    constexpr int SPW = 32 / NBITS;          // samples per output word = channels per row word
+    constexpr int RW = TR_CB / 4;            // words per tile row (32)
+    constexpr int WPC = TR_TS / SPW;         // output words per channel per tile
+    __shared__ __attribute__((aligned(16))) uint32_t tile[TR_TS * RW];
+
+    const size_t t0 = (size_t)blockIdx.x * TR_TS;
+    const size_t b0 = (size_t)blockIdx.y * TR_CB;
+    const int tid = threadIdx.x;
+    auto slot = [](int t, int d) { return t * RW + (d ^ ((t / SPW) & (RW - 1))); };
+
+    const bool words = ((in_stride | reinterpret_cast<uintptr_t>(in)) & 3u) == 0;
+    if (words && t0 + TR_TS <= nsamps && b0 + TR_CB <= row_bytes) {
+        // interior tile: every load in bounds, all issued before the first is
+        // waited on (thread tid takes word tid % 32 of rows tid / 32 + 8k)
+        constexpr int PER = TR_TS * RW / 256;
+        const int d = tid % RW, tl0 = tid / RW;
+        const uint8_t* src = in + (t0 + tl0) * in_stride + b0 + 4 * d;
+        uint32_t v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            v[k] = *reinterpret_cast<const uint32_t*>(src + (size_t)(8 * k) * in_stride);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) tile[slot(tl0 + 8 * k, d)] = v[k];
+    } else {
+        for (int i = tid; i < TR_TS * RW; i += 256) {
+            const int tl = i / RW, d = i % RW;
+            const size_t t = t0 + tl, b = b0 + 4 * (size_t)d;
+            uint32_t v = 0;
+            if (t < nsamps) {
+                if (words && b + 3 < row_bytes) {
+                    v = *reinterpret_cast<const uint32_t*>(in + t * in_stride + b);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (b + q < row_bytes) v |= (uint32_t)in[t * in_stride + b + q] << (8 * q);
+                }
+            }
+            tile[slot(tl, d)] = v;
+        }
+    }
+    __syncthreads();
+
+    const size_t c0 = (size_t)blockIdx.y * (TR_CB * 8 / NBITS);
+    const size_t w0 = t0 / SPW;
+    for (int i = tid; i < RW * WPC; i += 256) {
+        const int wl = i % WPC, d = i / WPC;
+        const size_t cb = c0 + (size_t)d * SPW;
+        if (cb >= nchans || w0 + wl >= row_words) continue;
+        uint32_t R[SPW];
+#pragma unroll
+        for (int r = 0; r < SPW; ++r) R[r] = tile[slot(wl * SPW + r, d)];
+        // element q of R[r] (bits q*NBITS) is channel cb+q at sample SPW*wl+r;
+        // after the stages it is channel cb+r at sample SPW*wl+q
+#pragma unroll
+        for (int B = SPW / 2; B >= 1; B /= 2) {
+            const int sh = B * NBITS;
+            const uint32_t m = low_blocks_mask(sh);
+#pragma unroll
+            for (int r = 0; r < SPW; ++r) {
+                if (r & B) continue;
+                const uint32_t a = R[r], b = R[r + B];
+                R[r] = (a & m) | ((b << sh) & ~m);
+                R[r + B] = ((a >> sh) & m) | (b & ~m);
+            }
+        }
+        uint32_t* o = out + cb * row_words + w0 + wl;
+        if (cb + SPW <= nchans) {
+#pragma unroll
+            for (int q = 0; q < SPW; ++q) o[q * row_words] = R[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < SPW; ++q)
+                if (cb + q < nchans) o[q * row_words] = R[q];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Generic channel sum
 // ---------------------------------------------------------------------------
@@ -488,9 +590,9 @@ extern "C" hipError_t dd_tdd_transpose(const uint8_t* in, size_t nsamps, size_t 
     const size_t rb = (in_stride < row_bytes) ? in_stride : row_bytes;
     switch (nbits) {
     case 1:  hipLaunchKernelGGL(tdd_transpose_k<1>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
-    case 2:  hipLaunchKernelGGL(tdd_transpose_k<2>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
-    case 4:  hipLaunchKernelGGL(tdd_transpose_k<4>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
-    case 8:  hipLaunchKernelGGL(tdd_transpose_k<8>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 2:  hipLaunchKernelGGL(tdd_transpose_reg_k<2>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 4:  hipLaunchKernelGGL(tdd_transpose_reg_k<4>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
+    case 8:  hipLaunchKernelGGL(tdd_transpose_reg_k<8>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
     case 16: hipLaunchKernelGGL(tdd_transpose_k<16>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
     case 32: hipLaunchKernelGGL(tdd_transpose_k<32>, grid, dim3(256), 0, stream, in, nsamps, in_stride, rb, nchans, out, row_words); break;
     default: return hipErrorInvalidValue;
