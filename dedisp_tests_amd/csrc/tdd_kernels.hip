@@ -24,8 +24,15 @@
 #ifndef TDD_EARLY_READS
 #define TDD_EARLY_READS 1
 #endif
+#ifndef TDD_STAGE_CHANNELS
+#define TDD_STAGE_CHANNELS 2
+#endif
 
 namespace {
+
+// a word of global memory: loads through such pointers are global (vector
+// memory only) rather than flat
+typedef const __attribute__((address_space(1))) uint32_t gword;
 
 // ---------------------------------------------------------------------------
 // Epilogue (src/tdd/dedisperse/dedisperse_kernel.cuh:45-87)
@@ -301,6 +308,8 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     constexpr uint32_t M8 = (NBITS == 4) ? 0x0F0F0F0Fu : (NBITS == 2) ? 0x03030303u : 0x01010101u;
 
     extern __shared__ uint32_t lds[];
+    // window staging: SC channels of SW words per lane in flight per wave
+    constexpr int SC = TDD_STAGE_CHANNELS, SW = 4;
 
     // XCD-aware mapping: workgroups are dispatched round-robin over the 8 XCDs;
     // give each XCD a contiguous range of (time tile, DM block) pairs with the
@@ -441,12 +450,40 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     for (uint32_t j0 = 0; j0 < nact; j0 += cc_max) {
         const uint32_t cc = min(cc_max, nact - j0);
         __syncthreads();
-        // Stage the delayed windows of channels j0 .. j0+cc-1.
-        for (uint32_t jj = wave; jj < cc; jj += WAVES) {
-            const uint32_t j = j0 + jj;
-            const uint32_t* src = rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
-            uint32_t* dst = lds + jj * W;
-            for (uint32_t w = lane; w < W; w += 64) dst[w] = src[w];
+        // Stage the delayed windows of channels j0 .. j0+cc-1: a wave takes
+        // channels wave, wave+WAVES, ...; with windows of at most SW*64 words
+        // all loads of SC channels are issued before the first is stored.
+        if (W <= SW * 64) {
+            for (uint32_t jb = wave; jb < cc; jb += SC * WAVES) {
+                uint32_t v[SC][SW];
+#pragma unroll
+                for (int u = 0; u < SC; ++u) {
+                    const uint32_t jj = jb + u * WAVES;
+                    if (jj >= cc) break;
+                    const uint32_t j = j0 + jj;
+                    const gword* src = (const gword*)rows + (size_t)chan[j] * row_words + tw0 +
+                                       wstart[wrow + j];
+#pragma unroll
+                    for (int m = 0; m < SW; ++m)
+                        if (lane + 64u * m < W) v[u][m] = src[lane + 64 * m];
+                }
+#pragma unroll
+                for (int u = 0; u < SC; ++u) {
+                    const uint32_t jj = jb + u * WAVES;
+                    if (jj >= cc) break;
+                    uint32_t* dst = lds + jj * W;
+#pragma unroll
+                    for (int m = 0; m < SW; ++m)
+                        if (lane + 64u * m < W) dst[lane + 64 * m] = v[u][m];
+                }
+            }
+        } else {
+            for (uint32_t jj = wave; jj < cc; jj += WAVES) {
+                const uint32_t j = j0 + jj;
+                const uint32_t* src = rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
+                uint32_t* dst = lds + jj * W;
+                for (uint32_t w = lane; w < W; w += 64) dst[w] = src[w];
+            }
         }
         __syncthreads();
 
