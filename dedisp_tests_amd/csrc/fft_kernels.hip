@@ -290,7 +290,17 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     const int tq = hi ? (base ? T3 - base : T3 / 2) : base;
     const bool self0 = (tq == 0), selfh = (tq == T3 / 2);
     float2 q[A];
-    fft3_core<A, B, C, -1>(ph, lds, load, q, NoPrep(), -1, tq);
+    if (!period && nbits == 8 && t0 + 2 * (size_t)M <= nvalid) {
+        // interior 8-bit chunk: every pair in bounds, no branch between the
+        // loads, so all of a thread's loads are in flight together
+        auto interior = [&](int n) -> float2 {
+            const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t0 + 2 * (size_t)n);
+            return make_float2(((float)(pair & 0xff) - mid) * s, ((float)(pair >> 8) - mid) * s);
+        };
+        fft3_core<A, B, C, -1>(ph, lds, interior, q, NoPrep(), -1, tq);
+    } else {
+        fft3_core<A, B, C, -1>(ph, lds, load, q, NoPrep(), -1, tq);
+    }
     // the value of the lane i ^ 32
     auto other_half = [&](float x) -> float {
         const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
