@@ -310,6 +310,7 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     extern __shared__ uint32_t lds[];
     // window staging: SC channels of SW words per lane in flight per wave
     constexpr int SC = TDD_STAGE_CHANNELS, SW = 4;
+    constexpr bool REGSTAGE = 64 * G + 1 <= SW * 64 && !NEED32;
 
     // XCD-aware mapping: workgroups are dispatched round-robin over the 8 XCDs;
     // give each XCD a contiguous range of (time tile, DM block) pairs with the
@@ -453,7 +454,13 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         // Stage the delayed windows of channels j0 .. j0+cc-1: a wave takes
         // channels wave, wave+WAVES, ...; with windows of at most SW*64 words
         // all loads of SC channels are issued before the first is stored.
-        if (W <= SW * 64) {
+        // Only variants whose windows can be that short carry that path, and
+        // not those with 32-bit lanes, where its registers would cost a wave
+        // per SIMD.
+        bool staged = false;
+        if constexpr (REGSTAGE) {
+          if (W <= SW * 64) {
+            staged = true;
             for (uint32_t jb = wave; jb < cc; jb += SC * WAVES) {
                 uint32_t v[SC][SW];
 #pragma unroll
@@ -477,7 +484,9 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
                         if (lane + 64u * m < W) dst[lane + 64 * m] = v[u][m];
                 }
             }
-        } else {
+          }
+        }
+        if (!staged) {
             for (uint32_t jj = wave; jj < cc; jj += WAVES) {
                 const uint32_t j = j0 + jj;
                 const uint32_t* src = rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
