@@ -24,15 +24,14 @@
 #ifndef TDD_EARLY_READS
 #define TDD_EARLY_READS 1
 #endif
-#ifndef TDD_STAGE_CHANNELS
-#define TDD_STAGE_CHANNELS 2
-#endif
 
 namespace {
 
 // a word of global memory: loads through such pointers are global (vector
 // memory only) rather than flat
 typedef const __attribute__((address_space(1))) uint32_t gword;
+// a word of LDS (the destination operand of global -> LDS loads)
+typedef __attribute__((address_space(3))) uint32_t lword;
 
 // ---------------------------------------------------------------------------
 // Epilogue (src/tdd/dedisperse/dedisperse_kernel.cuh:45-87)
@@ -308,9 +307,6 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     constexpr uint32_t M8 = (NBITS == 4) ? 0x0F0F0F0Fu : (NBITS == 2) ? 0x03030303u : 0x01010101u;
 
     extern __shared__ uint32_t lds[];
-    // window staging: SC channels of SW words per lane in flight per wave
-    constexpr int SC = TDD_STAGE_CHANNELS, SW = 4;
-    constexpr bool REGSTAGE = 64 * G + 1 <= SW * 64 && !NEED32;
 
     // XCD-aware mapping: workgroups are dispatched round-robin over the 8 XCDs;
     // give each XCD a contiguous range of (time tile, DM block) pairs with the
@@ -451,48 +447,17 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
     for (uint32_t j0 = 0; j0 < nact; j0 += cc_max) {
         const uint32_t cc = min(cc_max, nact - j0);
         __syncthreads();
-        // Stage the delayed windows of channels j0 .. j0+cc-1: a wave takes
-        // channels wave, wave+WAVES, ...; with windows of at most SW*64 words
-        // all loads of SC channels are issued before the first is stored.
-        // Only variants whose windows can be that short carry that path, and
-        // not those with 32-bit lanes, where its registers would cost a wave
-        // per SIMD.
-        bool staged = false;
-        if constexpr (REGSTAGE) {
-          if (W <= SW * 64) {
-            staged = true;
-            for (uint32_t jb = wave; jb < cc; jb += SC * WAVES) {
-                uint32_t v[SC][SW];
-#pragma unroll
-                for (int u = 0; u < SC; ++u) {
-                    const uint32_t jj = jb + u * WAVES;
-                    if (jj >= cc) break;
-                    const uint32_t j = j0 + jj;
-                    const gword* src = (const gword*)rows + (size_t)chan[j] * row_words + tw0 +
-                                       wstart[wrow + j];
-#pragma unroll
-                    for (int m = 0; m < SW; ++m)
-                        if (lane + 64u * m < W) v[u][m] = src[lane + 64 * m];
-                }
-#pragma unroll
-                for (int u = 0; u < SC; ++u) {
-                    const uint32_t jj = jb + u * WAVES;
-                    if (jj >= cc) break;
-                    uint32_t* dst = lds + jj * W;
-#pragma unroll
-                    for (int m = 0; m < SW; ++m)
-                        if (lane + 64u * m < W) dst[lane + 64 * m] = v[u][m];
-                }
-            }
-          }
-        }
-        if (!staged) {
-            for (uint32_t jj = wave; jj < cc; jj += WAVES) {
-                const uint32_t j = j0 + jj;
-                const uint32_t* src = rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
-                uint32_t* dst = lds + jj * W;
-                for (uint32_t w = lane; w < W; w += 64) dst[w] = src[w];
-            }
+        // Stage the delayed windows of channels j0 .. j0+cc-1, a wave taking
+        // channels wave, wave+WAVES, ...: global -> LDS loads (LDS address =
+        // the wave-uniform row base + 4*lane), which need no registers and no
+        // wait before the barrier, so every staging load of the wave is in
+        // flight at once.
+        for (uint32_t jj = wave; jj < cc; jj += WAVES) {
+            const uint32_t j = j0 + jj;
+            const gword* src = (const gword*)rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
+            lword* dst = (lword*)(lds + jj * W);
+            for (uint32_t w0 = 0; w0 < W; w0 += 64)
+                if (lane + w0 < W) __builtin_amdgcn_global_load_lds(src + w0 + lane, dst + w0, 4, 0, 0);
         }
         __syncthreads();
 
