@@ -454,10 +454,13 @@ __global__ __launch_bounds__(WAVES * 64) void tdd_lds_k(
         // flight at once.
         for (uint32_t jj = wave; jj < cc; jj += WAVES) {
             const uint32_t j = j0 + jj;
+            // wave-uniform row start; lane is the only per-lane part of the
+            // address, and only the last, partial 64 words need a bound
             const gword* src = (const gword*)rows + (size_t)chan[j] * row_words + tw0 + wstart[wrow + j];
             lword* dst = (lword*)(lds + jj * W);
-            for (uint32_t w0 = 0; w0 < W; w0 += 64)
-                if (lane + w0 < W) __builtin_amdgcn_global_load_lds(src + w0 + lane, dst + w0, 4, 0, 0);
+            uint32_t w0 = 0;
+            for (; w0 + 64 <= W; w0 += 64) __builtin_amdgcn_global_load_lds(src + w0 + lane, dst + w0, 4, 0, 0);
+            if (lane < W - w0) __builtin_amdgcn_global_load_lds(src + w0 + lane, dst + w0, 4, 0, 0);
         }
         __syncthreads();
 
