@@ -96,14 +96,16 @@ dd_tdd_tiling default_tiling(unsigned nbits)
     // 1000 DMs, 16 samples/lane: 44.4 ms (8 samples/lane, 4 DMs: 50.8 ms);
     // 2-bit, 512 ch x 1024 DMs: 12.8 ms (13.3 ms); 1-bit, 512 ch x 1024 DMs:
     // 11.9 ms (13.7 ms). 4-bit with 32-channel chunks: 28.8 ms (16: 29.9,
-    // 64: 42.5).
+    // 64: 42.5). With the windows staged by global-to-LDS loads the 4-bit
+    // optimum moved: 4 DMs per wave, 4 waves, 32-channel chunks 25.7 ms
+    // (2 DMs x 8 waves: 26.1; 16-channel chunks 26.4, 64: 42.5).
     t.dms_per_wave = 2;
     t.waves = 4;
     t.chunk_channels = 16;
     switch (nbits) {
     case 1: t.samples_per_lane = 32; break;
     case 2: t.samples_per_lane = 16; break;
-    case 4: t.samples_per_lane = 16; t.waves = 8; t.chunk_channels = 32; break;
+    case 4: t.samples_per_lane = 16; t.dms_per_wave = 4; t.chunk_channels = 32; break;
     default: t.samples_per_lane = 16; break;
     }
     return t;
