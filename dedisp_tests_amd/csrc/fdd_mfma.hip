@@ -68,6 +68,11 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 #ifndef DD_MFMA_WAVES
 #define DD_MFMA_WAVES 8
 #endif
+// float32 form: complex products with three real products (Gauss), two bins
+// per workgroup (three accumulator sets)
+#ifndef DD_MFMA_GAUSS
+#define DD_MFMA_GAUSS 1
+#endif
 
 constexpr int WAVES = DD_MFMA_WAVES;   // waves per workgroup, one 32-DM tile each
 #ifndef DD_MFMA_NB
@@ -78,8 +83,16 @@ constexpr int KS = 16;                 // channels per K-step
 constexpr int NJ = 32;                 // chunks per tile
 constexpr int SLAB = NB * 2 * NJ * 8;  // elements of one plane of a K-step: [b][g][j][s]
 constexpr int NT = WAVES * 64;
-constexpr int HG = NB * NJ * KS / (4 * NT);  // groups of 4 channels a thread stages
-static_assert(HG >= 1 && HG * 4 * NT == NB * NJ * KS, "staging layout");
+
+// Bins per workgroup, plane size and channels a thread stages, per form.
+template <int PREC>
+struct Blk {
+    static constexpr bool GAUSS = PREC == 0 && DD_MFMA_GAUSS;
+    static constexpr int NB = GAUSS ? 2 : ::NB;
+    static constexpr int SLAB = NB * 2 * NJ * 8;
+    static constexpr int CPT = NB * NJ * KS / NT;
+    static_assert(CPT * NT == NB * NJ * KS && (CPT == 4 || (GAUSS && CPT == 2)), "staging layout");
+};
 
 template <int PREC>
 struct Form;
@@ -153,6 +166,10 @@ void fdd_mfma_k(
     float2* __restrict__ Y, size_t Yp, uint32_t nbb, uint32_t ndb)
 {
     using F = Form<PREC>;
+    // per form: these hide the namespace-scope defaults
+    constexpr bool GAUSS = Blk<PREC>::GAUSS;
+    constexpr int NB = Blk<PREC>::NB;
+    constexpr int SLAB = Blk<PREC>::SLAB;
     constexpr int BUF = F::PLANES * SLAB * F::ELEM;  // bytes of one staged K-step
     __shared__ __attribute__((aligned(16))) char xs[2 * BUF];
 #if !DD_MFMA_TRANS
@@ -190,7 +207,7 @@ void fdd_mfma_k(
     // K-step.
     // A thread stages bin sb of chunk sj for CPT consecutive channels sc0 ..
     // (consecutive threads take consecutive bins: contiguous in X).
-    constexpr int CPT = 4 * HG;
+    constexpr int CPT = Blk<PREC>::CPT;
     static_assert(NT % (NB * NJ) == 0 && (NT / (NB * NJ)) * CPT == KS, "staging covers a K-step");
     const int sb = tid % NB, sj = (tid / NB) % NJ, sc0 = (tid / (NB * NJ)) * CPT;
     const uint32_t sk = k0 + sb, sch = ch0 + sj;
@@ -205,14 +222,28 @@ void fdd_mfma_k(
     const size_t rs = (size_t)nchunk * Xp;
     const float2* xsrc = X + ((size_t)sc0 * nchunk + min(sch, nchunk - 1)) * Xp + min(sk, M);
 
-    float2 xv[4 * HG];
+    float2 xv[CPT];
     auto load_x = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4 * HG; ++q) xv[q] = xsrc[q * rs];
+        for (int q = 0; q < CPT; ++q) xv[q] = xsrc[q * rs];
     };
     auto store_x = [&](char* buf) {
+      if constexpr (GAUSS) {
+        // planes Xr, Xi - Xr, -(Xr + Xi) of two channels
+        float xr[2], xi[2];
 #pragma unroll
-      for (int u = 0; u < HG; ++u) {
+        for (int q = 0; q < 2; ++q) {
+            xr[q] = xv[q].x * xs_scale;
+            xi[q] = xv[q].y * xs_scale;
+        }
+        float2* p = reinterpret_cast<float2*>(buf) + s_off(0) / 2;
+        p[0] = make_float2(xr[0], xr[1]);
+        p[SLAB / 2] = make_float2(xi[0] - xr[0], xi[1] - xr[1]);
+        p[2 * SLAB / 2] = make_float2(-(xr[0] + xi[0]), -(xr[1] + xi[1]));
+        return;
+      }
+#pragma unroll
+      for (int u = 0; u < CPT / 4; ++u) {
         float xr[4], xi[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -257,11 +288,19 @@ void fdd_mfma_k(
     const uint16_t* qrow = q16 + (size_t)min(dm, ndm - 1) * qpitch + 8 * g;
     const int b_off = (g * NJ + i) * 8;  // B operand: chunk j = lane%32 (same index as i)
 
-    f32x16 acc_r[NB], acc_i[NB];
+    // Gauss: acc_k = sum (Pr + Pi) Xr, acc_r = -sum Pi (Xr + Xi),
+    // acc_i = sum Pr (Xi - Xr); Y = (acc_k + acc_r, acc_k + acc_i)
+    f32x16 acc_r[NB], acc_i[NB], acc_k[GAUSS ? NB : 1];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc_r[b][r] = acc_i[b][r] = 0.f;
+    if constexpr (GAUSS) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc_k[b][r] = 0.f;
+    }
 
     // Phasors of bin k0 and the per-bin steps for this lane's 8 channels. The
     // second half of the waves computes them for the next K-step at the end
@@ -381,12 +420,22 @@ void fdd_mfma_k(
                 *reinterpret_cast<float4*>(xi + 4) = X[SLAB / 4 + 1];
                 *reinterpret_cast<float4*>(xn) = X[2 * SLAB / 4];
                 *reinterpret_cast<float4*>(xn + 4) = X[2 * SLAB / 4 + 1];
+                if constexpr (GAUSS) {
+                    // planes: xr = Xr, xi = Xi - Xr, xn = -(Xr + Xi)
 #pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xr[s], acc_r[b], 0, 0, 0);
-                    acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xn[s], acc_r[b], 0, 0, 0);
-                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xi[s], acc_i[b], 0, 0, 0);
-                    acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xr[s], acc_i[b], 0, 0, 0);
+                    for (int s = 0; s < 8; ++s) {
+                        acc_k[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x + p[s].y, xr[s], acc_k[b], 0, 0, 0);
+                        acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xn[s], acc_r[b], 0, 0, 0);
+                        acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xi[s], acc_i[b], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xr[s], acc_r[b], 0, 0, 0);
+                        acc_r[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xn[s], acc_r[b], 0, 0, 0);
+                        acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].x, xi[s], acc_i[b], 0, 0, 0);
+                        acc_i[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(p[s].y, xr[s], acc_i[b], 0, 0, 0);
+                    }
                 }
             } else if constexpr (PREC == 1) {
                 const bf16x8* X = reinterpret_cast<const bf16x8*>(bb);
@@ -501,6 +550,15 @@ void fdd_mfma_k(
     const uint32_t ch = ch0 + i;
     if (ch >= nchunk) return;
     const bool all_bins = k0 + NB - 1 <= M;
+    if constexpr (GAUSS) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                acc_r[b][r] += acc_k[b][r];
+                acc_i[b][r] += acc_k[b][r];
+            }
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const uint32_t row = 8 * (r / 4) + 4 * g + (r % 4);
@@ -535,16 +593,19 @@ extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp,
         (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
         return hipErrorInvalidValue;
     const size_t M = nfft / 2;
-    const uint32_t nbb = (uint32_t)((M + 1 + NB - 1) / NB);
     const uint32_t ndb = (uint32_t)((ndm + WAVES * 32 - 1) / (WAVES * 32));
     const uint32_t ncg = (uint32_t)((nchunk + NJ - 1) / NJ);
-    const size_t nblocks = (size_t)nbb * ndb * ncg;
-    if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;
     const float yscale = 1.0f / xscale;
+    // bin blocks of the form's workgroup
 #define DD_MFMA(P)                                                                                  \
-    hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q16,     \
-                       qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536,          \
-                       (uint32_t)nfft, xscale, yscale, Y, Yp, nbb, ndb)
+    {                                                                                               \
+        const uint32_t nbb = (uint32_t)((M + Blk<P>::NB) / Blk<P>::NB);                             \
+        const size_t nblocks = (size_t)nbb * ndb * ncg;                                             \
+        if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;                              \
+        hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q16, \
+                           qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536,      \
+                           (uint32_t)nfft, xscale, yscale, Y, Yp, nbb, ndb);                        \
+    }
     switch (prec) {
     case 0: DD_MFMA(0); break;
     case 1: DD_MFMA(1); break;
