@@ -3,8 +3,8 @@
 // For every Fourier bin k the dedispersed spectra are a complex GEMM
 //   Y_k[dm][chunk] = sum_c P_k[dm][c] X_k[c][chunk],  P_k[dm][c] = e^{+2 pi i k d(dm,c)/nfft}
 // (the phase rotation of src/fdd/dedisperse/fdd_kernel.cuh:226-235 with the
-// truncated integer delay d). A workgroup owns NB consecutive bins, 32 chunks
-// and WAVES*32 DMs; a wave owns a 32 (DM) x 32 (chunk) tile of every bin and
+// truncated integer delay d). A workgroup owns NB consecutive bins (4; 2 for
+// the float32 form), 32 chunks and WAVES*32 DMs; a wave owns a 32 (DM) x 32 (chunk) tile of every bin and
 // walks the channels 16 at a time (one K-step).
 //
 // Phasors (A operand), generated in registers:
@@ -20,12 +20,16 @@
 //   59.9 vs 56.9 ms, fp16 36.6 vs 34.4 ms), the random-address table reads
 //   costing LDS time and bank conflicts.
 // Spectra (B operand): X_k[c][chunk] for the K-step, staged in LDS in the
-//   planes of the form (float32: re, im, -im; bfloat16: re hi, re lo, im hi,
+//   planes of the form (float32: re, im - re, -(re + im); bfloat16: re hi, re lo, im hi,
 //   im lo and the negated im planes; float16: re, im, -im), formed on load from
 //   the real spectra of the forward transform. The next K-step's
 //   spectra and delays are loaded into registers while the current one is
 //   multiplied; one barrier per K-step (two LDS buffers).
-// Complex MAC as real MFMAs: Yr += Pr Xr + Pi (-Xi), Yi += Pr Xi + Pi Xr.
+// Complex MAC as real MFMAs: Yr += Pr Xr + Pi (-Xi), Yi += Pr Xi + Pi Xr;
+// the float32 form uses three products (Gauss): K = (Pr + Pi) Xr,
+// A = Pi (-(Xr + Xi)), B = Pr (Xi - Xr), Yr = K + A, Yi = K + B, each summed in
+// its own accumulators (three sets, hence two bins per workgroup; with
+// DD_MFMA_GAUSS=0 it runs the four-product form on four bins).
 // Forms: float32 (v_mfma_f32_32x32x2_f32), bfloat16 (v_mfma_f32_32x32x16_bf16,
 // phasors and spectra split into truncated hi + rounded lo, products
 // Ph Xh + Ph Xl + Pl Xh), float16 (v_mfma_f32_32x32x16_f16, one product,
@@ -97,7 +101,7 @@ struct Blk {
 template <int PREC>
 struct Form;
 template <>
-struct Form<0> {  // float32: re, im, -im
+struct Form<0> {  // float32: re, im - re, -(re + im) (Gauss) or re, im, -im
     static constexpr int PLANES = 3, ELEM = 4;
 };
 // The bfloat16 and float16 forms stage re and im only when DD_MFMA_NEGA is
