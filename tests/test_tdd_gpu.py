@@ -148,6 +148,31 @@ def test_tdd_adv_strides_and_guru_subset(dd, orc):
     assert np.array_equal(out2, ref[5:12])
 
 
+@pytest.mark.parametrize("nbits", [2, 4, 8])
+@pytest.mark.parametrize("nch,pad", [(12, 3), (100, 1), (300, 0), (300, 2)])
+def test_tdd_ragged_channels_and_unaligned_strides(dd, orc, nbits, nch, pad):
+    # channel counts (filling whole bytes, as the input layout requires) that
+    # leave part of a sample row's last 32-bit word unused, and row strides
+    # that are not a multiple of 4 bytes (the transpose's byte loads); the
+    # padding bytes must be ignored
+    rng = np.random.default_rng(nch * 10 + nbits + pad)
+    nsamps = 3000 + 77
+    s = rng.integers(0, 1 << nbits, size=(nsamps, nch)).astype(np.uint32)
+    packed = dd.pack_samples(s, nbits)
+    rb = packed.shape[1]
+    padded = np.full((nsamps, rb + pad), 0xA5, np.uint8)
+    padded[:, :rb] = packed
+    dms = np.linspace(0, 40, 23, dtype=f32)
+    plan = _plan(dd, nch, 64e-6, 1581.8, -0.390625)
+    plan.set_dm_list(dms)
+    nsc = nsamps - plan.max_delay
+    out = np.zeros((len(dms), nsc), f32)
+    assert plan.execute_adv(nsamps, padded, nbits, rb + pad, out, 32, nsc * 4) == 0
+    delays = orc.delay_table(nch, f32(64e-6), f32(1581.8), f32(-0.390625))
+    ref = orc.tdd(nch, delays, None, dms, nsamps, plan.max_delay, packed, nbits, 32)
+    assert np.array_equal(out, ref)
+
+
 def test_tdd_validation_errors(dd):
     plan = _plan(dd, 64, 64e-6, 1581.8, -2.0)
     buf = np.zeros((100, 64), np.uint8)
