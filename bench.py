@@ -271,6 +271,39 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
             del h_out
         del d_out, plan
         torch.cuda.empty_cache()
+    if "fp32" in forms:
+        # the reference fork's default output: the whole-series Fourier
+        # coefficients, N = round_up(nsamps + max_delay, 16384) floats per DM
+        # (bins 0..N/2-1 as re/im), float32 contraction
+        plan = dd.Plan(NCH, DT, F0, DF, impl=dd.FDD)
+        plan.set_dm_list(dms)
+        N = plan.out_length(NSAMPS)
+        nsc = NSAMPS - plan.max_delay
+        d_out = torch.empty((2048, N), dtype=torch.float32, device="cuda")
+        plan.set_profiling(True)
+        assert plan.execute_raw(NSAMPS, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+        torch.cuda.synchronize()
+        nst = max(1, steps // 2)
+        prof = []
+        t0 = time.perf_counter()
+        for _ in range(nst):
+            assert plan.execute_raw(NSAMPS, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+            prof.append(plan.profile())
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / nst
+        got = d_out[sample_rows].cpu().numpy().astype(np.float64)
+        refF = oracle.fdd_fourier(NCH, otab, dms[sample_rows], NSAMPS, plan.max_delay, host_in)
+        out["fourier_fp32"] = {
+            "value": 2048 * nsc / el, "ms_per_step": 1e3 * el, "N": N,
+            "stages_ms": {"transpose": float(np.mean([p["unpack_ms"] for p in prof])),
+                          "forward_fft": float(np.mean([p["fft_ms"] for p in prof])),
+                          "contraction": float(np.mean([p["dedisperse_ms"] for p in prof])),
+                          "inverse_and_row_transforms": float(np.mean([p["ifft_ms"] for p in prof]))},
+            "rel_l2_vs_fp64": float(np.linalg.norm(got - refF) / np.linalg.norm(refF)),
+            "note": "default output mode of the reference fork (whole-series Fourier coefficients); "
+                    "value in DM x (nsamps - max_delay) per second like the other forms"}
+        del d_out, plan
+        torch.cuda.empty_cache()
     if "fp32" in out and "bf16" in out:
         out["speedup_bf16_vs_fp32"] = out["bf16"]["value"] / out["fp32"]["value"]
     if "fp32" in out and "fp16" in out:

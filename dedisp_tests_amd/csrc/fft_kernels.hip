@@ -524,18 +524,30 @@ __global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict
     if (k2 >= 8192) return;
     const float2* a = A + row * m * 8192 + k2;
     float2* z = Z + row * z_stride + k2;
-    for (size_t k1 = 0; k1 < m; ++k1) {
-        float2 acc = make_float2(0.f, 0.f);
-        size_t e = 0;  // (n1 * k1) mod m
+    // KB outputs k1 per pass over the inputs, so each input is loaded once per
+    // pass instead of once per output; every output still sums n1 = 0, 1, ...
+    // in order with the same operations
+    constexpr int KB = 16;
+    for (size_t k1b = 0; k1b < m; k1b += KB) {
+        float2 acc[KB];
+        uint32_t e[KB];  // (n1 * (k1b + j)) mod m
+#pragma unroll
+        for (int j = 0; j < KB; ++j) acc[j] = make_float2(0.f, 0.f), e[j] = 0;
         for (size_t n1 = 0; n1 < m; ++n1) {
-            const float2 w = twL[e * 8192];
             const float2 v = a[n1 * 8192];
-            acc.x = fmaf(v.x, w.x, fmaf(-v.y, w.y, acc.x));
-            acc.y = fmaf(v.x, w.y, fmaf(v.y, w.x, acc.y));
-            e += k1;
-            if (e >= m) e -= m;
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                if (k1b + j >= m) break;
+                const float2 w = twL[(size_t)e[j] * 8192];
+                acc[j].x = fmaf(v.x, w.x, fmaf(-v.y, w.y, acc[j].x));
+                acc[j].y = fmaf(v.x, w.y, fmaf(v.y, w.x, acc[j].y));
+                e[j] += (uint32_t)(k1b + j);
+                if (e[j] >= m) e[j] -= (uint32_t)m;
+            }
         }
-        z[k1 * 8192] = acc;
+#pragma unroll
+        for (int j = 0; j < KB; ++j)
+            if (k1b + j < m) z[(k1b + j) * 8192] = acc[j];
     }
 }
 
