@@ -504,7 +504,14 @@ __global__ __launch_bounds__(512) void fourier_step_a_k(
 {
     using Sh = FftShape<16, 16, 32>;
     __shared__ float lds[Sh::LDS_FLOATS];
-    const size_t n1 = blockIdx.x, row = blockIdx.y;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
+    // each XCD takes a contiguous range of (row, n1), so the neighbouring
+    // columns n1, n1+1, ... that share the strided input's cache lines run on
+    // one L2
+    const uint32_t nb = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
+    const uint32_t xcd = bid & 7u, per = nb >> 3, rem = nb & 7u;
+    const uint32_t lin = xcd * per + min(xcd, rem) + (bid >> 3);
+    const size_t n1 = lin % m, row = lin / m;
     const float* src = S + row * s_stride;
     float2* dst = A + (row * m + n1) * 8192;
     auto load = [&](int n2) -> float2 {
