@@ -491,6 +491,8 @@ hipError_t launch_inverse_chunks(const float2* Y, size_t nrows, size_t row0, con
 // Real transform of length N = 16384*m per row (whole-series Fourier output).
 // Packed z[n] = s[2n] + i s[2n+1], L = N/2 = 8192*m points, four-step with
 // n = n1 + m*n2, k = k2 + 8192*k1:
+//   columns: z viewed as [n2][n1] transposed to [n1][n2] (tiles through LDS),
+//            so each column n1 is contiguous
 //   step A: 8192-point DFT over n2 for each (row, n1), times W_L^{n1 k2}
 //   step B: m-point DFT over n1 for each (row, k2)
 //   step C: real-input pairing X[k] = (Z[k] + conj Z[L-k])/2
@@ -498,9 +500,29 @@ hipError_t launch_inverse_chunks(const float2* Y, size_t nrows, size_t row0, con
 // twL[j] = e^{-2 pi i j/L}, twN[k] = e^{-2 pi i k/N} (j, k < L), built in double.
 // ---------------------------------------------------------------------------
 
+__global__ __launch_bounds__(256) void fourier_cols_k(const float2* __restrict__ Z, size_t z_stride, size_t m,
+                                                      float2* __restrict__ T, size_t t_stride)
+{
+    __shared__ float2 tile[32][33];
+    const size_t row = blockIdx.z;
+    const size_t n1_0 = (size_t)blockIdx.x * 32, n2_0 = (size_t)blockIdx.y * 32;
+    const float2* src = Z + row * z_stride;
+    float2* dst = T + row * t_stride;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const size_t n1 = n1_0 + tx;
+        if (n1 < m) tile[r][tx] = src[n1 + m * (n2_0 + r)];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const size_t n1 = n1_0 + r;
+        if (n1 < m) dst[n1 * 8192 + n2_0 + tx] = tile[tx][r];
+    }
+}
+
 __global__ __launch_bounds__(512) void fourier_step_a_k(
-    const float* __restrict__ S, size_t s_stride, size_t m, const float2* __restrict__ ph8192,
-    const float2* __restrict__ twL, float2* __restrict__ A)
+    const float2* __restrict__ T, size_t t_stride, size_t m, const float2* __restrict__ ph8192,
+    const float2* __restrict__ twL, float2* __restrict__ A, size_t a_stride)
 {
     using Sh = FftShape<16, 16, 32>;
     __shared__ float lds[Sh::LDS_FLOATS];
@@ -512,24 +534,30 @@ __global__ __launch_bounds__(512) void fourier_step_a_k(
     const uint32_t xcd = bid & 7u, per = nb >> 3, rem = nb & 7u;
     const uint32_t lin = xcd * per + min(xcd, rem) + (bid >> 3);
     const size_t n1 = lin % m, row = lin / m;
-    const float* src = S + row * s_stride;
-    float2* dst = A + (row * m + n1) * 8192;
-    auto load = [&](int n2) -> float2 {
-        const size_t n = n1 + m * (size_t)n2;
-        return make_float2(src[2 * n], src[2 * n + 1]);
-    };
+    const float2* src = T + row * t_stride + n1 * 8192;  // column n1, contiguous
+    float2* dst = A + row * a_stride + n1 * 8192;
+    auto load = [&](int n2) -> float2 { return src[n2]; };
     auto store = [&](int k2, float2 v) { dst[k2] = cmul(v, twL[n1 * (size_t)k2]); };
     fft3<16, 16, 32, -1>(ph8192, lds, load, store);
 }
 
-__global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict__ A, size_t m,
-                                                        const float2* __restrict__ twL,
+__global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict__ A, size_t a_stride,
+                                                        size_t m, const float2* __restrict__ twL,
                                                         float2* __restrict__ Z, size_t z_stride)
 {
+    // the m distinct twiddles W_m^e = twL[8192 e] (e < m), staged in LDS when
+    // they fit: each is then a broadcast LDS read rather than a scalar load
+    // waited on before its use
+    __shared__ float2 wt[4096];
+    const bool tw_lds = m <= 4096;
+    if (tw_lds) {
+        for (size_t i = threadIdx.x; i < m; i += 256) wt[i] = twL[i * 8192];
+        __syncthreads();
+    }
     const size_t k2 = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t row = blockIdx.y;
     if (k2 >= 8192) return;
-    const float2* a = A + row * m * 8192 + k2;
+    const float2* a = A + row * a_stride + k2;
     float2* z = Z + row * z_stride + k2;
     // KB outputs k1 per pass over the inputs, so each input is loaded once per
     // pass instead of once per output; every output still sums n1 = 0, 1, ...
@@ -545,7 +573,7 @@ __global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict
 #pragma unroll
             for (int j = 0; j < KB; ++j) {
                 if (k1b + j >= m) break;
-                const float2 w = twL[(size_t)e[j] * 8192];
+                const float2 w = tw_lds ? wt[e[j]] : twL[(size_t)e[j] * 8192];
                 acc[j].x = fmaf(v.x, w.x, fmaf(-v.y, w.y, acc[j].x));
                 acc[j].y = fmaf(v.x, w.y, fmaf(v.y, w.x, acc[j].y));
                 e[j] += (uint32_t)(k1b + j);
@@ -582,23 +610,36 @@ extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t n
                                        float2* work, float2* out, size_t out_stride,
                                        hipStream_t stream)
 {
-    if (N % 16384) return hipErrorInvalidValue;
+    if (N % 16384 || s_stride % 2 || s_stride < N) return hipErrorInvalidValue;
     const size_t m = N / 16384, L = N / 2;
-    // the 8192-point engine uses the 16384-point phasor table (ph[2j] = W_8192^-j)
-    hipLaunchKernelGGL(fourier_step_a_k, dim3((unsigned)m, (unsigned)nrows),
-                       dim3(512), 0, stream, S, s_stride, m, ph16384, twL, work);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // step B writes the packed spectrum over the rows of S (L complex = N floats per row)
-    float2* Z = reinterpret_cast<float2*>(const_cast<float*>(S));
-    const size_t z_stride = s_stride / 2;
-    hipLaunchKernelGGL(fourier_step_b_k, dim3(32, (unsigned)nrows), dim3(256), 0, stream, work, m,
-                       twL, Z, z_stride);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fourier_step_c_k, dim3((unsigned)((L + 255) / 256), (unsigned)nrows), dim3(256),
-                       0, stream, Z, z_stride, L, twN, out, out_stride);
-    return hipGetLastError();
+    // S rows hold L complex each (pitch s_stride/2); work holds nrows x L complex
+    float2* Sc = reinterpret_cast<float2*>(const_cast<float*>(S));
+    const size_t sp = s_stride / 2;
+    // rows in slices that keep every grid dimension within its limit
+    constexpr size_t kRows = 32768;
+    for (size_t r0 = 0; r0 < nrows; r0 += kRows) {
+        const unsigned nr = (unsigned)std::min(kRows, nrows - r0);
+        float2* Sr = Sc + r0 * sp;
+        float2* Wr = work + r0 * L;
+        // columns: S -> work; step A: work -> S; step B: S -> work; step C: work -> out
+        hipLaunchKernelGGL(fourier_cols_k, dim3((unsigned)((m + 31) / 32), 8192 / 32, nr), dim3(256), 0, stream,
+                           Sr, sp, m, Wr, L);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        // the 8192-point engine uses the 16384-point phasor table (ph[2j] = W_8192^-j)
+        hipLaunchKernelGGL(fourier_step_a_k, dim3((unsigned)m, nr), dim3(512), 0, stream, Wr, L, m, ph16384,
+                           twL, Sr, sp);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(fourier_step_b_k, dim3(32, nr), dim3(256), 0, stream, Sr, sp, m, twL, Wr, L);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(fourier_step_c_k, dim3((unsigned)((L + 255) / 256), nr), dim3(256), 0, stream, Wr, L,
+                           L, twN, out + r0 * out_stride, out_stride);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Zero fill by a kernel (16-byte stores), usable inside captured graphs.
