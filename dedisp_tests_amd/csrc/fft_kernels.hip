@@ -559,6 +559,42 @@ __global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict
     if (k2 >= 8192) return;
     const float2* a = A + row * a_stride + k2;
     float2* z = Z + row * z_stride + k2;
+    if (tw_lds && m % 2 == 0) {
+        // even m: one radix-2 split, E[k] / O[k] = the h-point DFTs (h = m/2)
+        // of the even / odd n1, X[k] = E[k] + W_m^k O[k], X[k+h] = E[k] - W_m^k O[k];
+        // W_h^e = W_m^{2e}. KH outputs k per pass over the inputs.
+        constexpr int KH = 8;
+        const uint32_t h = (uint32_t)(m / 2);
+        for (uint32_t kb = 0; kb < h; kb += KH) {
+            float2 E[KH], O[KH];
+            uint32_t e[KH];  // (p * (kb + j)) mod h
+#pragma unroll
+            for (int j = 0; j < KH; ++j) E[j] = O[j] = make_float2(0.f, 0.f), e[j] = 0;
+            for (uint32_t p = 0; p < h; ++p) {
+                const float2 ve = a[(size_t)(2 * p) * 8192], vo = a[(size_t)(2 * p + 1) * 8192];
+#pragma unroll
+                for (int j = 0; j < KH; ++j) {
+                    if (kb + j >= h) break;
+                    const float2 w = wt[2 * e[j]];
+                    E[j].x = fmaf(ve.x, w.x, fmaf(-ve.y, w.y, E[j].x));
+                    E[j].y = fmaf(ve.x, w.y, fmaf(ve.y, w.x, E[j].y));
+                    O[j].x = fmaf(vo.x, w.x, fmaf(-vo.y, w.y, O[j].x));
+                    O[j].y = fmaf(vo.x, w.y, fmaf(vo.y, w.x, O[j].y));
+                    e[j] += kb + j;
+                    if (e[j] >= h) e[j] -= h;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < KH; ++j) {
+                const uint32_t k = kb + j;
+                if (k >= h) break;
+                const float2 t = cmul(O[j], wt[k]);
+                z[(size_t)k * 8192] = cadd(E[j], t);
+                z[(size_t)(k + h) * 8192] = csub(E[j], t);
+            }
+        }
+        return;
+    }
     // KB outputs k1 per pass over the inputs, so each input is loaded once per
     // pass instead of once per output; every output still sums n1 = 0, 1, ...
     // in order with the same operations
