@@ -34,7 +34,8 @@ def filterbank(nsamps: int, nchans: int, nbits: int, mean: float, sigma: float, 
     return s.astype(np.uint32)
 
 
-# Workload geometries (the BASELINE configurations).
+# Workload geometries: 256 channels x 65536 8-bit samples; 1024 channels x 2^20
+# samples, 4-bit (generated DM list) and 8-bit (2048 linear DM trials).
 CFG1 = dict(nchans=256, f0=1581.8, df=-0.390625, dt=64e-6, nsamps=65536, nbits=8,
             mean=128.0, sigma=20.0, dms=("linear", 0.0, 1.0, 256))
 CFG2 = dict(nchans=1024, f0=1530.0, df=-0.3125, dt=54.6e-6, nsamps=1 << 20, nbits=4,
