@@ -60,6 +60,18 @@ dedisp_error guarded(F&& f)
     return DEDISP_NO_ERROR;
 }
 
+// A push writes chunk output samples per DM, which holds while the plan's DM
+// list is the one the stream was created with and an FDD plan still writes
+// time series (Fourier output is N floats per DM and would overrun the
+// stream's output rows).
+void check_plan_unchanged(const dedisp_amd_stream_struct& s)
+{
+    if (s.plan->get_max_delay() != s.hist || s.plan->get_dm_count() != s.ndm)
+        DD_THROW(DEDISP_INVALID_PLAN);
+    if (auto* f = dynamic_cast<const dedisp::FDDGPUPlan*>(s.plan))
+        if (f->get_output_mode() != DEDISP_AMD_FDD_TIMESERIES) DD_THROW(DEDISP_INVALID_PLAN);
+}
+
 }  // namespace
 
 extern "C" {
@@ -127,8 +139,7 @@ dedisp_error dedisp_amd_stream_push(dedisp_amd_stream s, dedisp_size beam, const
     if (!chunk || !out) return DEDISP_INVALID_POINTER;
     if (beam >= s->nbeams) return DEDISP_INVALID_POINTER;
     return guarded([&] {
-        if (s->plan->get_max_delay() != s->hist || s->plan->get_dm_count() != s->ndm)
-            DD_THROW(DEDISP_INVALID_PLAN);  // DM list changed since creation
+        check_plan_unchanged(*s);
         const auto t0 = std::chrono::steady_clock::now();
         dedisp::GPUContext& ctx = s->plan->context();
         ctx.make_current();
@@ -172,8 +183,7 @@ dedisp_error dedisp_amd_stream_push_batch(dedisp_amd_stream s, dedisp_size n, co
         seen[beams[i]] = 1;
     }
     return guarded([&] {
-        if (s->plan->get_max_delay() != s->hist || s->plan->get_dm_count() != s->ndm)
-            DD_THROW(DEDISP_INVALID_PLAN);
+        check_plan_unchanged(*s);
         const auto t0 = std::chrono::steady_clock::now();
         dedisp::GPUContext& ctx = s->plan->context();
         ctx.make_current();

@@ -204,3 +204,23 @@ def test_stream_push_batch_host_and_errors(dd):
     assert st.push_batch_raw([1, 1], [buf, buf], [out, out]) == dd.DEDISP_INVALID_POINTER  # repeated beam
     assert st.push_batch_raw([3], [buf], [out]) == dd.DEDISP_INVALID_POINTER               # out of range
     assert st.push_batch_raw([], [], []) == 0
+
+
+def test_stream_refuses_fdd_plan_switched_to_fourier(dd):
+    """A stream switches its FDD plan to time series; switching the plan back to
+    Fourier output afterwards (N floats per DM, longer than a chunk row) makes
+    every push fail with DEDISP_INVALID_PLAN before anything runs, and the
+    stream works again once time series is restored."""
+    nch = 64
+    plan = dd.Plan(nch, 64e-6, 1581.8, -2.0, impl=dd.FDD)
+    plan.set_dm_list(np.linspace(0, 40, 5, dtype=f32))
+    st = dd.Stream(plan, nbeams=2, chunk_nsamps=2000, in_nbits=8, out_nbits=32)
+    buf = np.full((2000, nch), 128, np.uint8)
+    out = np.zeros((5, 2000), np.float32)
+    plan.fdd_set_output(dd.FDD_FOURIER)
+    assert st.push_raw(0, buf, out) == dd.DEDISP_INVALID_PLAN
+    assert st.push_batch_raw([0, 1], [buf, buf], [out, out.copy()]) == dd.DEDISP_INVALID_PLAN
+    assert st.stats()["pushes"] == 0
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    assert st.push_raw(0, buf, out) == 0
+    st.close()
