@@ -41,6 +41,8 @@ DEDISP, TDD, FDD = 0, 1, 2
 USE_DEFAULT, HOST_POINTERS, DEVICE_POINTERS, WAIT, ASYNC = 0, 1 << 1, 1 << 2, 1 << 3, 1 << 4
 # dedisp_amd_fdd_output / precision
 FDD_FOURIER, FDD_TIMESERIES = 0, 1
+# bytes of the fixed device scratch behind dedisp_amd_execute_noalloc (dedisp_amd.h)
+NOALLOC_SCRATCH_BYTES = 128 << 20
 FDD_FP32, FDD_BF16, FDD_FP16 = 0, 1, 2
 
 
@@ -107,6 +109,8 @@ _SIG = [
     ("dedisp_amd_execute_batch", C.c_int, [C.c_void_p, C.c_ulong, C.POINTER(C.c_ulong), C.POINTER(C.c_void_p),
                                            C.c_ulong, C.POINTER(C.c_void_p), C.c_ulong, C.c_uint]),
     ("dedisp_amd_autotune", C.c_int, [C.c_void_p, C.c_ulong, C.c_ulong, C.c_ulong, C.c_void_p]),
+    ("dedisp_amd_execute_noalloc", C.c_int,
+     [C.c_void_p, C.c_ulong, C.c_void_p, C.c_ulong, C.c_void_p, C.c_ulong, C.c_uint]),
     ("dedisp_select_implementation", C.c_int, [C.c_int]),
     ("dedisp_create_plan", C.c_int, [C.POINTER(C.c_void_p), C.c_ulong, C.c_float, C.c_float, C.c_float]),
     ("dedisp_destroy_plan", None, [C.c_void_p]),
@@ -334,6 +338,12 @@ class Plan:
 
     def execute_raw(self, nsamps, in_ptr, in_nbits, out_ptr, out_nbits, flags=0) -> int:
         return lib().dedisp_execute(self._h, nsamps, _ptr(in_ptr), in_nbits, _ptr(out_ptr), out_nbits, flags)
+
+    def execute_noalloc_raw(self, nsamps, in_ptr, in_nbits, out_ptr, out_nbits, flags=0) -> int:
+        """dedisp_amd_execute_noalloc: device pointers, the library's fixed
+        scratch instead of device allocations; returns the dedisp_error code."""
+        return lib().dedisp_amd_execute_noalloc(self._h, nsamps, _ptr(in_ptr), in_nbits, _ptr(out_ptr),
+                                                out_nbits, flags)
 
     def execute_adv(self, nsamps, in_ptr, in_nbits, in_stride, out_ptr, out_nbits, out_stride, flags=0) -> int:
         return lib().dedisp_execute_adv(self._h, nsamps, _ptr(in_ptr), in_nbits, in_stride, _ptr(out_ptr),

@@ -8,6 +8,7 @@
 // compatibility (set_gulp_size / get_gulp_size) and bounds the host-pointer
 // staging chunk.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -27,7 +28,15 @@ struct TDDState {
     DeviceBuffer d_weight;  // [nact] killmask weight (generic path)
     DeviceBuffer d_wstart;  // [ndmblk][nact] staged window start (fast path)
 
-    // cache key of the device tables
+    // host copies of the tables (what the device buffers above hold once
+    // uploaded): fast path delay codes or generic [ndm][nact] delays, channel
+    // indices, weights, window starts
+    std::vector<uint32_t> h_tab, h_wstart;
+    std::vector<int32_t> h_chan, h_weight;
+    bool uploaded = false;   // d_code / d_chan / d_weight / d_wstart hold the host tables
+    uint64_t table_uid = 0;  // process-unique id of the current host tables
+
+    // cache key of the tables
     unsigned long version = ~0ul;
     unsigned nbits = 0;
     size_t first_dm = ~size_t(0), dm_count = 0;
@@ -126,11 +135,16 @@ TDDPlan::~TDDPlan() {}
 
 void TDDPlan::set_gulp_size(size_type gulp_size) { m_gulp_size = gulp_size; }
 
-// Device tables for input width nbits and DMs first_dm_idx .. +dm_count:
-// active channels, integer delays, and for the packed-lane kernel the window
-// starts and delay codes. Cached per DM list / killmask / width / DM range /
-// tiling; independent of the series length.
-void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count)
+namespace {
+std::atomic<uint64_t> g_table_uid{0};
+}
+
+// Tables for input width nbits and DMs first_dm_idx .. +dm_count: active
+// channels, integer delays, and for the packed-lane kernel the window starts
+// and delay codes. Built on the host and cached per DM list / killmask / width
+// / DM range / tiling (independent of the series length); with upload they are
+// also copied to the plan's device buffers.
+void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count, bool upload)
 {
     GPUContext& ctx = *m_ctx;
     TDDState& st = *m_state;
@@ -226,24 +240,12 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
             }
         }
 
-        st.d_chan.reserve(std::max<size_t>(nact, 1) * 4);
-        st.d_weight.reserve(std::max<size_t>(nact, 1) * 4);
-        if (nact) {
-            DD_HIP(hipMemcpyAsync(st.d_chan.ptr, chan.data(), nact * 4, hipMemcpyHostToDevice, stream));
-            DD_HIP(hipMemcpyAsync(st.d_weight.ptr, weight.data(), nact * 4, hipMemcpyHostToDevice, stream));
-        }
-        const std::vector<uint32_t>& tab = fast ? code : d;
-        // 64 spare entries: the channel-sum kernel may prefetch one pair of
-        // codes past its last channel
-        st.d_code.reserve((tab.size() + 64) * 4);
-        if (!tab.empty())
-            DD_HIP(hipMemcpyAsync(st.d_code.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
-        if (fast) {
-            st.d_wstart.reserve(wstart.size() * 4);
-            DD_HIP(hipMemcpyAsync(st.d_wstart.ptr, wstart.data(), wstart.size() * 4,
-                                  hipMemcpyHostToDevice, stream));
-        }
-        DD_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
+        st.h_chan = std::move(chan);
+        st.h_weight = std::move(weight);
+        st.h_tab = fast ? std::move(code) : std::move(d);
+        st.h_wstart = fast ? std::move(wstart) : std::vector<uint32_t>();
+        st.uploaded = false;
+        st.table_uid = ++g_table_uid;
 
         st.version = m_tables_version;
         st.nbits = nbits;
@@ -255,7 +257,28 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
         st.dmax = dmax;
         st.max_wstart = max_wstart;
     }
-
+    if (upload && !st.uploaded) {
+        const size_t nact = st.nact;
+        st.d_chan.reserve(std::max<size_t>(nact, 1) * 4);
+        st.d_weight.reserve(std::max<size_t>(nact, 1) * 4);
+        if (nact) {
+            DD_HIP(hipMemcpyAsync(st.d_chan.ptr, st.h_chan.data(), nact * 4, hipMemcpyHostToDevice, stream));
+            DD_HIP(hipMemcpyAsync(st.d_weight.ptr, st.h_weight.data(), nact * 4, hipMemcpyHostToDevice, stream));
+        }
+        // 64 spare entries: the channel-sum kernel may prefetch one pair of
+        // codes past its last channel
+        st.d_code.reserve((st.h_tab.size() + 64) * 4);
+        if (!st.h_tab.empty())
+            DD_HIP(hipMemcpyAsync(st.d_code.ptr, st.h_tab.data(), st.h_tab.size() * 4, hipMemcpyHostToDevice,
+                                  stream));
+        if (st.fast) {
+            st.d_wstart.reserve(st.h_wstart.size() * 4);
+            DD_HIP(hipMemcpyAsync(st.d_wstart.ptr, st.h_wstart.data(), st.h_wstart.size() * 4,
+                                  hipMemcpyHostToDevice, stream));
+        }
+        DD_HIP(hipStreamSynchronize(stream));
+        st.uploaded = true;
+    }
 }
 
 // Words per channel row of the transposed input for nsamps samples with nsc
@@ -437,6 +460,7 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
     }
 }
 
+
 namespace {
 
 dd_tdd_epilogue make_epilogue(unsigned nbits, size_t out_nbits, size_t nchans)
@@ -453,6 +477,86 @@ dd_tdd_epilogue make_epilogue(unsigned nbits, size_t out_nbits, size_t nchans)
 }
 
 }  // namespace
+
+void TDDPlan::execute_noalloc(size_type nsamps, const byte_type* in, size_type in_nbits, byte_type* out,
+                              size_type out_nbits, unsigned flags, void* scratch, size_type scratch_bytes,
+                              unsigned long long* scratch_tag)
+{
+    // execute()'s checks (src/tdd/TDDPlan.cpp:167-195) for device pointers and
+    // unpadded rows
+    if (!in || !out || !scratch || !scratch_tag) DD_THROW(DEDISP_INVALID_POINTER);
+    if (m_dm_count == 0) DD_THROW(DEDISP_NO_DM_LIST_SET);
+    if (nsamps < m_max_delay) DD_THROW(DEDISP_TOO_FEW_NSAMPS);
+    if ((flags & DEDISP_ASYNC) && (flags & DEDISP_WAIT)) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (flags & DEDISP_HOST_POINTERS) DD_THROW(DEDISP_INVALID_FLAG_COMBINATION);
+    if (in_nbits != 1 && in_nbits != 2 && in_nbits != 4 && in_nbits != 8 && in_nbits != 16 &&
+        in_nbits != 32)
+        DD_THROW(DEDISP_UNSUPPORTED_IN_NBITS);
+    if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
+    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    const size_t nsc = nsamps - m_max_delay;
+    if (nsc == 0) return;
+
+    GPUContext& ctx = *m_ctx;
+    TDDState& st = *m_state;
+    ctx.make_current();
+    hipStream_t stream = ctx.exec;
+    const unsigned nbits = (unsigned)in_nbits;
+    prepare_tables(nbits, 0, m_dm_count, /*upload=*/false);
+    const size_t row_words = row_words_for(nbits, nsamps, nsc);
+
+    // scratch layout: delay codes (64 spare entries), channels, weights,
+    // window starts, then the transposed rows; 256-byte aligned pieces
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = used;
+        used += round_up(std::max<size_t>(bytes, 4), 256);
+        return at;
+    };
+    const size_t o_code = carve((st.h_tab.size() + 64) * 4);
+    const size_t o_chan = carve(st.nact * 4);
+    const size_t o_weight = carve(st.nact * 4);
+    const size_t o_wstart = carve(st.h_wstart.size() * 4);
+    const size_t o_rows = carve(m_nchans * row_words * 4);
+    if (used > scratch_bytes) DD_THROW(DEDISP_MEM_ALLOC_FAILED);
+    uint8_t* const base = static_cast<uint8_t*>(scratch);
+    auto* d_code = reinterpret_cast<uint32_t*>(base + o_code);
+    auto* d_chan = reinterpret_cast<int32_t*>(base + o_chan);
+    auto* d_weight = reinterpret_cast<int32_t*>(base + o_weight);
+    auto* d_wstart = reinterpret_cast<uint32_t*>(base + o_wstart);
+    auto* d_rows = reinterpret_cast<uint32_t*>(base + o_rows);
+
+    // tables: uploaded only when the scratch holds another plan's (or an
+    // older) set; the rows that follow them are rewritten by every call
+    if (*scratch_tag != st.table_uid) {
+        *scratch_tag = 0;
+        if (st.nact) {
+            DD_HIP(hipMemcpyAsync(d_chan, st.h_chan.data(), st.nact * 4, hipMemcpyHostToDevice, stream));
+            DD_HIP(hipMemcpyAsync(d_weight, st.h_weight.data(), st.nact * 4, hipMemcpyHostToDevice, stream));
+        }
+        if (!st.h_tab.empty())
+            DD_HIP(hipMemcpyAsync(d_code, st.h_tab.data(), st.h_tab.size() * 4, hipMemcpyHostToDevice, stream));
+        if (!st.h_wstart.empty())
+            DD_HIP(hipMemcpyAsync(d_wstart, st.h_wstart.data(), st.h_wstart.size() * 4, hipMemcpyHostToDevice,
+                                  stream));
+        DD_HIP(hipStreamSynchronize(stream));
+        *scratch_tag = st.table_uid;
+    }
+
+    // the transpose writes every word of every row (zero beyond the data), so
+    // nothing in the scratch needs clearing
+    const size_t row_bytes = (m_nchans * nbits + 7) / 8;
+    DD_HIP(dd_tdd_transpose(in, nsamps, row_bytes, nbits, m_nchans, d_rows, row_words, stream));
+    const size_t out_stride = nsc * (out_nbits / 8);
+    const dd_tdd_epilogue ep = make_epilogue(nbits, out_nbits, m_nchans);
+    if (st.fast)
+        DD_HIP(dd_tdd_dedisperse_lds(d_rows, row_words, nbits, d_code, d_chan, d_wstart, st.nact, m_dm_count, nsc,
+                                     out, out_stride, ep, st.tiling, stream));
+    else
+        DD_HIP(dd_tdd_dedisperse_generic(d_rows, row_words, nbits, d_code, d_chan, d_weight, st.nact, m_dm_count,
+                                         nsc, out, out_stride, ep, stream));
+    if (!(flags & DEDISP_ASYNC)) DD_HIP(hipStreamSynchronize(stream));
+}
 
 void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byte_type* const* in,
                             size_type in_nbits, byte_type* const* out, size_type out_nbits,

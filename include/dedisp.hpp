@@ -165,8 +165,20 @@ public:
     double autotune(size_type nsamps, size_type in_nbits, size_type out_nbits,
                     dedisp_amd_tuning* chosen);
 
+    /* execute() on device pointers with every device buffer it needs (the
+     * transposed input rows and the delay tables) carved out of the caller's
+     * scratch of scratch_bytes, allocating no device memory
+     * (dedisp_amd_execute_noalloc). *scratch_tag names the tables the scratch
+     * holds: they are uploaded only when it differs from this plan's current
+     * tables, and it is updated. Throws DEDISP_MEM_ALLOC_FAILED when rows and
+     * tables do not fit. */
+    void execute_noalloc(size_type nsamps, const byte_type* in, size_type in_nbits,
+                         byte_type* out, size_type out_nbits, unsigned flags,
+                         void* scratch, size_type scratch_bytes, unsigned long long* scratch_tag);
+
 private:
-    void prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count);
+    void prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count,
+                        bool upload = true);
     size_t row_words_for(unsigned nbits, size_t nsamps, size_t nsc) const;
 
     dedisp_size m_gulp_size;

@@ -218,6 +218,32 @@ typedef struct {
 dedisp_error dedisp_amd_autotune(dedisp_plan plan, dedisp_size nsamps, dedisp_size in_nbits,
                                  dedisp_size out_nbits, dedisp_amd_tuning* result);
 
+/* ---------------------------------------------------------------------------
+ * Allocation-free execution. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).
+ *
+ * dedisp_execute on a TDD or legacy plan with device pointers (in: nsamps
+ * unpadded time-major rows of nchans*in_nbits/8 bytes; out: DM-major unpadded
+ * rows of nsamps - max_delay outputs), giving the same output, but every
+ * device buffer it needs (the transposed input rows and the delay tables)
+ * lives in one fixed device array of DEDISP_AMD_NOALLOC_SCRATCH_BYTES per
+ * device, held from the library's load until the process ends; no device
+ * memory is allocated or freed. It needs
+ *     nchans * 4 * row_words + the tables (about ndm * nchans * 4 bytes)
+ * with row_words ~ (nsamps + max_delay) * in_nbits / 32 plus a tile; calls
+ * whose rows and tables do not fit return DEDISP_MEM_ALLOC_FAILED before any
+ * device work. The tables are uploaded from the host only when the array holds
+ * another plan's (or an older DM list's) tables. flags: DEDISP_DEVICE_POINTERS
+ * (implied), DEDISP_ASYNC (return without waiting for the plan's stream);
+ * DEDISP_HOST_POINTERS is refused. Every call on one device shares that
+ * device's array: calls that overlap on one device (several host threads, or
+ * DEDISP_ASYNC calls of different plans) race. */
+#define DEDISP_AMD_NOALLOC_SCRATCH_BYTES (128ul << 20)
+dedisp_error dedisp_amd_execute_noalloc(dedisp_plan plan, dedisp_size nsamps, const dedisp_byte* in,
+                                        dedisp_size in_nbits, dedisp_byte* out, dedisp_size out_nbits,
+                                        unsigned flags);
+
 #ifdef __cplusplus
 }
 #endif
