@@ -1,12 +1,15 @@
 // TDDPlan: host side of the time-domain dedispersion path.
 //
-// Reference: src/tdd/TDDPlan.cpp:43-487. The reference streams 65536-sample
-// gulps through pinned double buffers because its device copy of the input is
-// gulp-sized; here the whole series is made resident in HBM (288 GB per GPU),
-// transposed once, and every output sample of every DM is produced by one
-// launch of the channel-sum kernel. The gulp size is kept as plan state for API
-// compatibility (set_gulp_size / get_gulp_size) and bounds the host-pointer
-// staging chunk.
+// Reference: src/tdd/TDDPlan.cpp:43-487. The reference streams gulps of
+// m_gulp_size (65536) output samples through pinned double buffers because its
+// device copy of the input is gulp-sized; here the whole series is made
+// resident in HBM (288 GB per GPU), transposed once, and every output sample
+// of every DM is produced by one launch of the channel-sum kernel. Gulps are
+// used when the caller sets a gulp size (set_gulp_size), when the resident
+// buffers of a series would exceed 80 % of the free device memory, or when a
+// series has 2^31 samples or more: the series is then processed in gulps of
+// output samples, each reading its max_delay samples of overlap again
+// (src/tdd/TDDPlan.cpp:225-263, 354-420).
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
@@ -48,6 +51,9 @@ struct TDDState {
     size_t dmax = 0;        // largest delay (samples) over the DMs in use
     size_t max_wstart = 0;  // fast path
     size_t row_words = 0;
+
+    // the caller chose the gulp size (set_gulp_size): execute in gulps of it
+    bool gulp_explicit = false;
 
     // tilings chosen by the autotuner, per input width (waves == 0: none)
     dd_tdd_tiling tuned[33] = {};
@@ -133,7 +139,12 @@ TDDPlan::TDDPlan(size_type nchans, float_type dt, float_type f0, float_type df, 
 
 TDDPlan::~TDDPlan() {}
 
-void TDDPlan::set_gulp_size(size_type gulp_size) { m_gulp_size = gulp_size; }
+void TDDPlan::set_gulp_size(size_type gulp_size)
+{
+    if (gulp_size == 0) DD_THROW(DEDISP_UNKNOWN_ERROR);
+    m_gulp_size = gulp_size;
+    m_state->gulp_explicit = true;
+}
 
 namespace {
 std::atomic<uint64_t> g_table_uid{0};
@@ -341,18 +352,62 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
     if (out_nbits != 8 && out_nbits != 16 && out_nbits != 32) DD_THROW(DEDISP_UNSUPPORTED_OUT_NBITS);
     if (first_dm_idx > m_dm_count || dm_count > m_dm_count - first_dm_idx)
         DD_THROW(DEDISP_UNKNOWN_ERROR);
-    if (nsamps >= (size_t(1) << 31)) DD_THROW(DEDISP_UNKNOWN_ERROR);
 
-    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
     const size_t nsc = nsamps - m_max_delay;
     if (nsc == 0 || dm_count == 0) return;
+
+    // Output samples per gulp: the caller's gulp size; else the whole series
+    // unless its resident buffers (rows, and for host pointers the staged input
+    // and output) exceed 80 % of the free device memory or it reaches 2^31
+    // samples (the kernels index samples in 32 bits).
+    const unsigned nbits = (unsigned)in_nbits;
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const double row_bytes_per_sample = (double)m_nchans * nbits / 8.0;
+    size_t gulp = nsc;
+    if (m_state->gulp_explicit) gulp = std::min(nsc, (size_t)m_gulp_size);
+    const size_t kMaxSpan = (size_t(1) << 31) - (size_t(1) << 20);  // samples per span, with tile rounding room
+    if (gulp + m_max_delay > kMaxSpan) {
+        if (m_max_delay + 65536 > kMaxSpan) DD_THROW(DEDISP_UNKNOWN_ERROR);
+        gulp = kMaxSpan - m_max_delay;
+    }
+    {
+        m_ctx->make_current();
+        size_t free_b = 0, total_b = 0;
+        DD_HIP(hipMemGetInfo(&free_b, &total_b));
+        auto need = [&](size_t g) {
+            const double span = (double)(g + m_max_delay);
+            double b = span * row_bytes_per_sample * 1.25;  // rows (tile rounding, 1-bit rows padded)
+            if (!device_ptrs) b += span * row_bytes_per_sample + (double)dm_count * g * (out_nbits / 8);
+            return b;
+        };
+        const double budget = 0.8 * (double)free_b;
+        while (gulp > 65536 && need(gulp) > budget) gulp = (gulp + 1) / 2;
+    }
+    m_ctx->timer.reset(m_ctx->profiling);
+    m_ctx->profile = dedisp_amd_profile{};
+    const size_t out_bytes = out_nbits / 8;
+    for (size_t o0 = 0; o0 < nsc; o0 += gulp) {
+        const size_t g = std::min(gulp, nsc - o0);
+        execute_span(g + m_max_delay, in + o0 * in_stride, in_nbits, in_stride, out + o0 * out_bytes, out_nbits,
+                     out_stride, first_dm_idx, dm_count, flags);
+    }
+}
+
+// One resident pass: nsamps input samples at in (nsamps - max_delay outputs
+// per DM at out), validated by execute_guru; stage times and work add to the
+// plan's profile.
+void TDDPlan::execute_span(size_type nsamps, const byte_type* in, size_type in_nbits, size_type in_stride,
+                           byte_type* out, size_type out_nbits, size_type out_stride, dedisp_size first_dm_idx,
+                           dedisp_size dm_count, unsigned flags)
+{
+    const dedisp_size out_bytes_per_sample = out_nbits / 8;
+    const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
+    const size_t nsc = nsamps - m_max_delay;
 
     GPUContext& ctx = *m_ctx;
     TDDState& st = *m_state;
     ctx.make_current();
     hipStream_t stream = ctx.exec;
-    ctx.timer.reset(ctx.profiling);
-    ctx.profile = dedisp_amd_profile{};
 
     const unsigned nbits = (unsigned)in_nbits;
     prepare_tables(nbits, first_dm_idx, dm_count);
@@ -456,7 +511,7 @@ void TDDPlan::execute_guru(size_type nsamps, const byte_type* in, size_type in_n
     if (!(device_ptrs && (flags & DEDISP_ASYNC))) {
         DD_HIP(hipStreamSynchronize(stream));
         ctx.timer.collect(ctx.profile);
-        ctx.profile.work_units = (double)dm_count * (double)nsc * (double)m_nchans;
+        ctx.profile.work_units += (double)dm_count * (double)nsc * (double)m_nchans;
     }
 }
 

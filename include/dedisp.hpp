@@ -131,6 +131,10 @@ public:
             int device_idx = 0);
     ~TDDPlan();
 
+    /* Output samples per gulp (reference default 65536). This library keeps
+     * the whole series resident and gulps only when a gulp size has been set
+     * here, when the series' device buffers would exceed 80 % of the free
+     * device memory, or from 2^31 samples on; 0 is refused. */
     void set_gulp_size(size_type gulp_size);
     dedisp_size get_gulp_size() const { return m_gulp_size; }
 
@@ -177,6 +181,9 @@ public:
                          void* scratch, size_type scratch_bytes, unsigned long long* scratch_tag);
 
 private:
+    void execute_span(size_type nsamps, const byte_type* in, size_type in_nbits, size_type in_stride,
+                      byte_type* out, size_type out_nbits, size_type out_stride, dedisp_size first_dm_idx,
+                      dedisp_size dm_count, unsigned flags);
     void prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_size dm_count,
                         bool upload = true);
     size_t row_words_for(unsigned nbits, size_t nsamps, size_t nsc) const;
@@ -217,14 +224,15 @@ public:
     dedisp_amd_fdd_output get_output_mode() const;
     void set_precision(dedisp_amd_fdd_precision precision);
     dedisp_amd_fdd_precision get_precision() const;
-    /* nfft = 0 selects the reference's automatic segment length
-     * (FDDGPUPlan.cpp:742-768); segmented mode writes time series. */
+    /* nfft = 0 selects the supported segment length (2^13 .. 2^18) with the
+     * smallest contraction volume (dedisp_amd_fdd_set_segmented); segmented
+     * mode writes time series. */
     void set_segmented(int enable, dedisp_size nfft);
 
     /* Output length per DM, in floats, for nsamps input samples. */
     dedisp_size output_length(dedisp_size nsamps) const;
 
-    /* Times the overlap-save segment lengths 2^13 .. 2^16 longer than
+    /* Times the overlap-save segment lengths 2^13 .. 2^18 longer than
      * max_delay on nsamps samples of in_nbits input (device scratch,
      * time-series output) and keeps the fastest (set_segmented(1, nfft)).
      * Returns its time in ms and the choice in *chosen. */

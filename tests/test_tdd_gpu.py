@@ -362,3 +362,43 @@ def test_host_pointers_rows_longer_than_a_transfer_slot(dd):
     assert plan.execute_raw(nsamps, d_in, nbits, d_out, 32, dd.DEVICE_POINTERS) == 0
     torch.cuda.synchronize()
     assert np.array_equal(out, d_out.cpu().numpy())
+
+
+@pytest.mark.parametrize("gulp,nbits,out_nbits", [(65536, 4, 32), (10007, 8, 16), (1 << 20, 2, 8)])
+def test_tdd_gulped_execution_matches_resident(dd, orc, gulp, nbits, out_nbits):
+    """set_gulp_size makes execute run in gulps of that many output samples
+    (the reference's TDDPlan.cpp:225-263 loop): outputs are bit-identical to the
+    resident single pass, on host and device pointers and with a DM subset
+    (execute_guru), the last gulp ragged."""
+    import torch
+    nch, nsamps = 256, 300001
+    rng = np.random.default_rng(gulp)
+    s = rng.integers(0, 1 << nbits, size=(nsamps, nch), dtype=np.uint32)
+    data = dd.pack_samples(s, nbits)
+    delays = orc.delay_table(nch, f32(64e-6), f32(1581.8), f32(-0.390625))
+    dms = np.linspace(0, 300, 97, dtype=f32)
+    ref_plan = _plan(dd, nch, 64e-6, 1581.8, -0.390625)
+    ref_plan.set_dm_list(dms)
+    ref = ref_plan.execute(data, nbits, out_nbits)
+    plan = _plan(dd, nch, 64e-6, 1581.8, -0.390625)
+    plan.set_dm_list(dms)
+    assert dd.lib().dedisp_set_gulp_size(plan.handle, gulp) == 0
+    assert dd.lib().dedisp_get_gulp_size(plan.handle) == gulp
+    got = plan.execute(data, nbits, out_nbits)
+    assert np.array_equal(got, ref)
+    # device pointers, DMs 10 .. 59 through execute_guru
+    nsc = nsamps - plan.max_delay
+    tdt = {8: torch.uint8, 16: torch.int16, 32: torch.float32}[out_nbits]
+    d_in = torch.from_numpy(data).cuda()
+    d_out = torch.zeros((50, nsc), dtype=tdt, device="cuda")
+    torch.cuda.synchronize()
+    ob = out_nbits // 8
+    assert plan.execute_guru(nsamps, d_in, nbits, data.shape[1], d_out, out_nbits, nsc * ob, 10, 50,
+                             dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    got_d = d_out.cpu().numpy().view(ref.dtype)
+    assert np.array_equal(got_d, ref[10:60])
+    # one row against the oracle as well
+    r = orc.tdd(nch, delays, None, dms, nsamps, plan.max_delay, data, nbits, out_nbits, first_dm=96, dm_count=1)
+    assert np.array_equal(got[96], r[0])
+    assert dd.lib().dedisp_set_gulp_size(plan.handle, 0) == dd.DEDISP_UNKNOWN_ERROR
