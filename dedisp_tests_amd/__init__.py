@@ -15,6 +15,7 @@ Importing never loads the oracle; the library must exist (build it with
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from pathlib import Path
 
@@ -256,9 +257,12 @@ class Plan:
         _check(l.dedisp_create_plan(C.byref(h), nchans, dt, f0, df), "dedisp_create_plan")
         self._h = h
         self.impl = impl
+        self._streams = weakref.WeakSet()  # streams over this plan: closed before it
 
     def close(self) -> None:
         if getattr(self, "_h", None):
+            for st in list(getattr(self, "_streams", ())):
+                st.close()
             lib().dedisp_destroy_plan(self._h)
             self._h = None
 
@@ -429,6 +433,7 @@ class Stream:
                "dedisp_amd_stream_create")
         self._h = h
         self.plan = plan
+        plan._streams.add(self)
         self.nbeams, self.chunk, self.in_nbits, self.out_nbits = nbeams, chunk_nsamps, in_nbits, out_nbits
 
     @property

@@ -23,6 +23,7 @@
 
 struct dedisp_amd_stream_struct {
     dedisp::GPUPlan* plan = nullptr;
+    int device = 0;
     size_t nbeams = 0, chunk = 0, in_nbits = 0, out_nbits = 0;
     size_t row_bytes = 0;  // bytes per input sample (all channels)
     size_t hist = 0;       // max_delay at creation
@@ -33,6 +34,7 @@ struct dedisp_amd_stream_struct {
     std::vector<double> lat_ms;
     double busy_s = 0;
     double samples = 0;  // input samples pushed (all beams)
+    int fill = 0;        // byte value of the samples before a beam's first push
 };
 
 namespace {
@@ -58,6 +60,22 @@ dedisp_error guarded(F&& f)
         return DEDISP_UNKNOWN_ERROR;
     }
     return DEDISP_NO_ERROR;
+}
+
+// The byte the history starts as: 0 for TDD (its outputs at t >= 0 never
+// read the history); for FDD the sample value nearest the zero level
+// (2^nbits - 1)/2 of every packed sample, since the history enters each
+// segment's transform and a constant offset there would raise the rounding
+// error of the outputs that follow.
+int history_fill(const dedisp::GPUPlan& p, size_t in_nbits)
+{
+    if (!dynamic_cast<const dedisp::FDDGPUPlan*>(&p)) return 0;
+    switch (in_nbits) {
+    case 1: return 0xff;  // value 1 (zero level 0.5)
+    case 2: return 0xaa;  // value 2 (zero level 1.5)
+    case 4: return 0x88;  // value 8 (zero level 7.5)
+    default: return 0x80; // value 128 (zero level 127.5)
+    }
 }
 
 // A push writes chunk output samples per DM, which holds while the plan's DM
@@ -111,6 +129,7 @@ dedisp_error dedisp_amd_stream_create(dedisp_amd_stream* stream, dedisp_plan pla
         if (need > free_b - free_b / 10) DD_THROW(DEDISP_MEM_ALLOC_FAILED);
 
         s->plan = gp;
+        s->device = gp->get_device();
         s->nbeams = nbeams;
         s->chunk = chunk_nsamps;
         s->in_nbits = in_nbits;
@@ -120,10 +139,11 @@ dedisp_error dedisp_amd_stream_create(dedisp_amd_stream* stream, dedisp_plan pla
         s->ndm = gp->get_dm_count();
         s->bufs = std::vector<dedisp::DeviceBuffer>(2 * nbeams);
         s->cur.assign(nbeams, 0);
+        s->fill = history_fill(*gp, in_nbits);
         const size_t bytes = (s->hist + s->chunk) * s->row_bytes;
         for (auto& b : s->bufs) {
             b.reserve(bytes);
-            DD_HIP(hipMemsetAsync(b.ptr, 0, bytes, ctx.exec));
+            DD_HIP(hipMemsetAsync(b.ptr, s->fill, bytes, ctx.exec));
         }
         DD_HIP(hipStreamSynchronize(ctx.exec));
     });
@@ -235,7 +255,7 @@ dedisp_error dedisp_amd_stream_reset(dedisp_amd_stream s)
     return guarded([&] {
         dedisp::GPUContext& ctx = s->plan->context();
         ctx.make_current();
-        for (auto& b : s->bufs) DD_HIP(hipMemsetAsync(b.ptr, 0, b.bytes, ctx.exec));
+        for (auto& b : s->bufs) DD_HIP(hipMemsetAsync(b.ptr, s->fill, b.bytes, ctx.exec));
         DD_HIP(hipStreamSynchronize(ctx.exec));
         std::fill(s->cur.begin(), s->cur.end(), 0);
         s->lat_ms.clear();
@@ -272,8 +292,8 @@ dedisp_size dedisp_amd_stream_history(const dedisp_amd_stream s) { return s ? s-
 void dedisp_amd_stream_destroy(dedisp_amd_stream s)
 {
     if (!s) return;
-    (void)hipSetDevice(s->plan->get_device());
-    (void)hipStreamSynchronize(s->plan->context().exec);
+    // the plan is not touched: it may already be gone
+    if (hipSetDevice(s->device) == hipSuccess) (void)hipDeviceSynchronize();
     delete s;
 }
 

@@ -65,11 +65,15 @@ dedisp_error dedisp_amd_get_profile(const dedisp_plan plan, dedisp_amd_profile* 
 
 dedisp_error dedisp_amd_fdd_set_output(dedisp_plan plan, dedisp_amd_fdd_output mode);
 dedisp_error dedisp_amd_fdd_set_precision(dedisp_plan plan, dedisp_amd_fdd_precision precision);
-/* enable=1 selects the time-segmented (overlap-save) path; nfft=0 picks the
- * reference's automatic segment length (src/fdd/FDDGPUPlan.cpp:742-768),
- * otherwise nfft must be a power of two larger than max_delay. The reference
- * selects this path with the USE_SEGMENTED environment variable, which is
- * honoured too. */
+/* enable=1 selects the time-segmented (overlap-save) path. nfft is the
+ * segment length: a power of two from 2^13 to 2^18 larger than max_delay + 3,
+ * or 0 for the supported length with the smallest contraction volume
+ * (chunks x (nfft/2 + 1)); the reference instead doubles nfft from 16384 until
+ * a fifth of it covers max_delay (src/fdd/FDDGPUPlan.cpp:756-762). The same
+ * choice picks the segments of the time-series and Fourier outputs when no
+ * length is set, so any max_delay below 2^18 - 3 samples is supported. The
+ * reference selects this path with the USE_SEGMENTED environment variable,
+ * which is honoured too. */
 dedisp_error dedisp_amd_fdd_set_segmented(dedisp_plan plan, int enable, dedisp_size nfft);
 /* Floats written per DM by dedisp_execute on an FDD plan for nsamps samples. */
 dedisp_size  dedisp_amd_fdd_output_length(const dedisp_plan plan, dedisp_size nsamps);
@@ -95,7 +99,8 @@ const char* dedisp_amd_version(void);
  * input on the device, so every push of one chunk yields chunk_nsamps output
  * samples per DM: after the n-th push (n = 0, 1, ...) of a beam the outputs are
  * the dedispersed samples t = n*chunk_nsamps - max_delay + i, i < chunk_nsamps,
- * of the beam's concatenated input (samples before the first push read as 0).
+ * of the beam's concatenated input (samples before the first push read as 0
+ * for TDD plans and as the zero level, 2^(in_nbits-1), for FDD plans).
  * Outputs for t >= 0 equal those of one dedisp_execute over the whole series.
  * The plan (TDD, legacy or FDD) must have its DM list set; an FDD plan is
  * switched to time-series output. max_delay is fixed at creation.
@@ -135,7 +140,7 @@ dedisp_error dedisp_amd_stream_push(dedisp_amd_stream stream, dedisp_size beam,
 dedisp_error dedisp_amd_stream_push_batch(dedisp_amd_stream stream, dedisp_size n,
                                           const dedisp_size* beams, const dedisp_byte* const* chunks,
                                           dedisp_byte* const* outs, unsigned flags);
-/* Zeroes every beam's history and the statistics. */
+/* Clears every beam's history (to its initial value) and the statistics. */
 dedisp_error dedisp_amd_stream_reset(dedisp_amd_stream stream);
 dedisp_error dedisp_amd_stream_get_stats(const dedisp_amd_stream stream,
                                          dedisp_amd_stream_stats* stats);
@@ -212,8 +217,8 @@ typedef struct {
 /* Times the plan's configurations on nsamps samples of in_nbits input in
  * device scratch (contents irrelevant to the timing) and keeps the fastest for
  * later executes: TDD / legacy plans every instantiated channel-sum tiling for
- * that width, FDD plans the overlap-save segment lengths 2^k >= 2 max_delay
- * (2^13 .. 2^18, time-series output). Requires the DM list; returns the
+ * that width, FDD plans the overlap-save segment lengths 2^13 .. 2^18 longer
+ * than max_delay (time-series output). Requires the DM list; returns the
  * choice in *result. */
 dedisp_error dedisp_amd_autotune(dedisp_plan plan, dedisp_size nsamps, dedisp_size in_nbits,
                                  dedisp_size out_nbits, dedisp_amd_tuning* result);
