@@ -9,8 +9,12 @@
 extern "C" {
 #endif
 
-// nfft values the on-chip transform engine handles.
+// Segment lengths the transforms handle: 2^13 .. 2^16 on chip, 2^17 and
+// 2^18 split in time (dd_fdd_forward_split / dd_fdd_inverse_split).
 int dd_fdd_fft_supported(size_t nfft);
+// Bytes of work buffer the split transforms need for ntransforms transforms
+// of length nfft (0 for nfft <= 65536).
+size_t dd_fdd_split_work(size_t nfft, size_t ntransforms);
 
 // Zeroes bytes at p (both multiples of 16) with a kernel: the form that may
 // be recorded in a captured graph.
@@ -67,6 +71,29 @@ hipError_t dd_fdd_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
                           const float2* ph, size_t nfft, float* out, size_t out_stride,
                           size_t good, size_t nsamps_out, hipStream_t stream);
 
+// dd_fdd_forward / dd_fdd_inverse for every supported nfft: up to 65536 the
+// on-chip kernels; 2^17 and 2^18 as R = nfft/65536 transforms of 32768
+// points per segment (decimation in time) and a radix-R combine, through
+// work (dd_fdd_split_work(nfft, nchan*nchunk) or (nrows*nchunk) bytes).
+// ph64k: e^{+2 pi i m/65536} (m < 65536), the on-chip engine's table.
+hipError_t dd_fdd_forward_split(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
+                                size_t period, const float* chan_scale, const float2* ph, const float2* ph64k,
+                                size_t nfft, float2* X, size_t nchan, size_t nchunk, size_t Xp, unsigned nbits,
+                                float2* work, hipStream_t stream);
+// packed != 0: Y holds the packed half-length spectrum Zu[0..M-1] of each
+// transform (dd_fdd_contract_mfma16 with emit_packed) instead of bins 0..M.
+hipError_t dd_fdd_inverse_split(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
+                                const float2* ph64k, size_t nfft, float* out, size_t out_stride, size_t good,
+                                size_t nsamps_out, float2* work, int packed, hipStream_t stream);
+// The chunk-table forms (dd_fdd_forward_chunks / dd_fdd_inverse_chunks) for
+// every supported nfft.
+hipError_t dd_fdd_forward_chunks_split(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
+                                       const float2* ph, const float2* ph64k, size_t nfft, float2* X, size_t nchan,
+                                       size_t Xp, unsigned nbits, float2* work, hipStream_t stream);
+hipError_t dd_fdd_inverse_chunks_split(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
+                                       size_t nchunk, size_t Yp, const float2* ph, const float2* ph64k, size_t nfft,
+                                       size_t good, float2* work, int packed, hipStream_t stream);
+
 // Real transforms of length N = 16384*m of nrows rows of S (float, row
 // stride s_stride >= N, even), bins 0..N/2-1 written as complex to out (row
 // stride out_stride complex). ph16384: e^{+2 pi i j/16384}; twL: e^{-2 pi i j/L},
@@ -87,17 +114,38 @@ hipError_t dd_fdd_contract_fp32(const float2* X, size_t Xp, const int32_t* delay
 // (v_mfma_f32_32x32x2_f32), 1 = bfloat16 MFMA with hi+lo split phasors and
 // spectra (three products), 2 = float16 MFMA (one product); float32
 // accumulation in all three. Same output as dd_fdd_contract_fp32; the delays
-// enter as phase steps q16[i][c] = (d[i][c] * 65536/nfft) mod 65536 (row pitch
-// qpitch, a multiple of 16 and >= nchan rounded up to 16, zero padded), and
-// tab65536 (read only by the kernel's table form, DD_MFMA_TRANS=0; the default
-// takes the hardware cosine and sine) holds e^{2 pi i 256 a/65536} (a < 256) followed by e^{2 pi i b/65536}
-// (b < 256). The spectra are multiplied by xscale before the MFMA inputs are
-// formed and Y by 1/xscale after (a power of two keeping float16 in range).
-// X must hold nchan rounded up to 16 channel rows, the extra rows zero.
-hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint16_t* q16,
-                                size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
-                                const float2* tab65536, size_t nfft,
+// enter as phase steps q[i][c] = d[i][c] mod nfft (uint32, row pitch qpitch, a
+// multiple of 16 and >= nchan rounded up to 16, zero padded). The spectra are
+// multiplied by xscale before the MFMA inputs are formed and Y by 1/xscale
+// after (a power of two keeping float16 in range). X must hold nchan rounded
+// up to 16 channel rows, the extra rows zero. nfft a power of two up to 2^18.
+hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint32_t* q,
+                                size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
                                 float xscale, float2* Y, size_t Yp, hipStream_t stream);
+
+// Phasor contraction on 16 x 16 matrix-core tiles with the real and
+// imaginary outputs of a chunk side by side along N. prec: 1 = bfloat16 with
+// hi+lo split phasors and spectra (three products), 2 = float16 (one
+// product); float32 accumulation. q: [ndm][qpitch] uint32 phase steps
+// d mod nfft (qpitch a multiple of 32, >= nchan rounded up to 32, zero
+// padded); X must hold nchan rounded up to 32 channel rows, the extra rows
+// zero. mirror != 0: each workgroup takes two bins k, k+1 <= nfft/4 and their
+// mirrors nfft/2 - k, whose phasors are (-1)^d conj(P_k). nfft a power of two
+// up to 2^18. Same output as dd_fdd_contract_mfma; with ph_packed
+// (e^{+2 pi i m/nfft}, mirror only) each row instead holds the packed
+// half-length spectrum the inverse transform starts from,
+// Zu[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/nfft} (Y[k] - conj Y[M-k]), k < M.
+// prec 0 = float32 (v_mfma_f32_16x16x4_f32). With dmlist (ndm floats) and
+// delay32 (the delay table, nchan rounded up to 32 entries, zero padded) the
+// integer delays are computed in the kernel as (int)(dm * delay[c]) and q is
+// not read.
+hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2* X, size_t Xp, const uint32_t* q,
+                                  size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
+                                  float xscale, float2* Y, size_t Yp, const float2* ph_packed,
+                                  const float* dmlist, const float* delay32, hipStream_t stream);
+// N-tiles (8 chunks each) per workgroup dd_fdd_contract_mfma16 uses for
+// nchunk chunks, and the number of chunk groups.
+int dd_fdd_mfma16_tiles(size_t nchunk, int prec, int* ntl, size_t* ngroups);
 
 #ifdef __cplusplus
 }

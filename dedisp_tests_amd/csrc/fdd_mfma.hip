@@ -8,17 +8,16 @@
 // walks the channels 16 at a time (one K-step).
 //
 // Phasors (A operand), generated in registers:
-//   phase in units of 2 pi/65536: q = (d * 65536/nfft) mod 65536 per (DM,
-//   channel), precomputed as uint16; the phase of bin k is k*q mod 65536
-//   (exact integer arithmetic). Per (DM, channel) and workgroup P at the first
-//   bin and the step W = e^{2 pi i q/65536} come from the hardware cosine and
-//   sine (v_cos_f32 / v_sin_f32 take the angle in revolutions, and m/65536 is
-//   exact in float); the following bins are P <- P*W (NB-1 complex products).
-//   With DD_MFMA_TRANS=0 P and W are instead T_hi[m>>8] * T_lo[m&255] from two
-//   256-entry LDS tables built in double (relative error ~1e-7): the same
-//   accuracy at the oracle (FP32 rel-L2 6.5e-7 vs 6.7e-7) but slower (bf16
-//   59.9 vs 56.9 ms, fp16 36.6 vs 34.4 ms), the random-address table reads
-//   costing LDS time and bank conflicts.
+//   the phase of bin k is the exact integer (k*q) mod nfft with
+//   q = d mod nfft per (DM, channel) (uint32), in units of 1/nfft; per (DM,
+//   channel) and workgroup P at the first bin and the step W = e^{2 pi i q/nfft}
+//   come from the hardware cosine and sine (v_cos_f32 / v_sin_f32 take the
+//   angle in revolutions, and m/nfft is exact in float for nfft <= 2^24); the
+//   following bins are P <- P*W (NB-1 complex products). An earlier form read
+//   P and W as T_hi[m>>8] * T_lo[m&255] from two 256-entry LDS tables built in
+//   double: the same accuracy at the oracle (FP32 rel-L2 6.5e-7 vs 6.7e-7) but
+//   slower (bf16 59.9 vs 56.9 ms, fp16 36.6 vs 34.4 ms), the random-address
+//   table reads costing LDS time and bank conflicts.
 // Spectra (B operand): X_k[c][chunk] for the K-step, staged in LDS in the
 //   planes of the form (float32: re, im - re, -(re + im); bfloat16: re hi, re lo, im hi,
 //   im lo and the negated im planes; float16: re, im, -im), formed on load from
@@ -53,9 +52,6 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 
-#ifndef DD_MFMA_LOPOLY
-#define DD_MFMA_LOPOLY 0
-#endif
 #ifndef DD_MFMA_NEGA
 #define DD_MFMA_NEGA 1
 #endif
@@ -64,9 +60,6 @@ typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 #endif
 #ifndef DD_MFMA_STAGGER
 #define DD_MFMA_STAGGER 1
-#endif
-#ifndef DD_MFMA_TRANS
-#define DD_MFMA_TRANS 1
 #endif
 
 #ifndef DD_MFMA_WAVES
@@ -151,11 +144,6 @@ __device__ __forceinline__ uint32_t pack_f16x2(float a, float b)
     return __builtin_bit_cast(uint32_t, v);
 }
 
-// Table form (DD_MFMA_TRANS=0):
-// e^{2 pi i m/65536} for m8 = 8*m (bits 3..18 used) is T_hi[m>>8] * T_lo[m&255];
-// the LDS byte offsets of the two entries are (m8 >> 8) & 0x7f8 and
-// 2048 + (m8 & 0x7f8).
-
 template <int PREC>
 __global__ __launch_bounds__(NT)
 #if DD_MFMA_OCC2
@@ -164,8 +152,8 @@ __attribute__((amdgpu_waves_per_eu(2, 2)))
 __attribute__((amdgpu_waves_per_eu(1, 1)))
 #endif
 void fdd_mfma_k(
-    const float2* __restrict__ X, size_t Xp, const uint16_t* __restrict__ q16, size_t qpitch,
-    uint32_t ndm, uint32_t nchan, uint32_t nchunk, const float2* __restrict__ tab65536,
+    const float2* __restrict__ X, size_t Xp, const uint32_t* __restrict__ q, size_t qpitch,
+    uint32_t ndm, uint32_t nchan, uint32_t nchunk,
     uint32_t nfft, float xscale, float yscale,
     float2* __restrict__ Y, size_t Yp, uint32_t nbb, uint32_t ndb)
 {
@@ -176,9 +164,6 @@ void fdd_mfma_k(
     constexpr int SLAB = Blk<PREC>::SLAB;
     constexpr int BUF = F::PLANES * SLAB * F::ELEM;  // bytes of one staged K-step
     __shared__ __attribute__((aligned(16))) char xs[2 * BUF];
-#if !DD_MFMA_TRANS
-    __shared__ __attribute__((aligned(16))) char tab[4096];  // T_hi[256], T_lo[256]
-#endif
 
     const uint32_t M = nfft / 2;
     const int tid = threadIdx.x;
@@ -197,11 +182,6 @@ void fdd_mfma_k(
     const uint32_t k0 = bblk * NB;
     const uint32_t ch0 = cgrp * NJ;
 
-#if !DD_MFMA_TRANS
-    for (int t = tid; t < 512; t += NT)
-        reinterpret_cast<float2*>(tab)[t] = tab65536[t];
-    __syncthreads();  // the table is read before the first K-step barrier
-#endif
 
     // ---- staging role: bin b, chunk j, channel group g, half h (4 channels) ----
     // Every load is unconditional and in bounds (X has its channel count
@@ -289,7 +269,9 @@ void fdd_mfma_k(
     // ---- compute role: DM row i, channel group g ----
     const int i = lane & 31, g = lane >> 5;
     const uint32_t dm = dblk * (WAVES * 32) + wave * 32 + i;
-    const uint16_t* qrow = q16 + (size_t)min(dm, ndm - 1) * qpitch + 8 * g;
+    const uint32_t* qrow = q + (size_t)min(dm, ndm - 1) * qpitch + 8 * g;
+    const uint32_t qmask = nfft - 1;
+    const float qinv = 1.0f / (float)nfft;
     const int b_off = (g * NJ + i) * 8;  // B operand: chunk j = lane%32 (same index as i)
 
     // Gauss: acc_k = sum (Pr + Pi) Xr, acc_r = -sum Pi (Xr + Xi),
@@ -310,67 +292,26 @@ void fdd_mfma_k(
     // second half of the waves computes them for the next K-step at the end
     // of the current one (the registers are free by then), so after each
     // barrier one wave of a SIMD issues MFMAs while the other generates
-    // phasors. With the LDS tables (DD_MFMA_TRANS=0) a K-step's table reads
-    // are issued before any product uses them, and with DD_MFMA_LOPOLY the
-    // bfloat16 and float16 forms evaluate the low
-    // factor e^{2 pi i j/65536} (j < 256, angle below 0.025) as
-    // (1 - t^2/2 + t^4/24, t - t^3/6), truncation error below 1e-10, instead
-    // of reading T_lo. That halves the random-address LDS reads but measured
-    // slower (bf16 61.4 -> 65.0 ms, fp16 37.4 -> 40.1 ms on 1024 channels x
-    // 2048 DMs x 23 chunks of 65536), so it is off by default.
-    constexpr bool lopoly = DD_MFMA_LOPOLY && PREC != 0;
-    auto t_lo = [](uint32_t j) -> float2 {
-        constexpr float c = 6.283185307179586f / 65536.f;
-        // (2^23 + j) c - 2^23 c in one rounding: j c for integer j < 2^23
-        const float t = __builtin_fmaf(__builtin_bit_cast(float, 0x4B000000u | j), c, -8388608.f * c);
-        const float t2 = t * t;
-        return make_float2(__builtin_fmaf(t2, __builtin_fmaf(t2, 1.f / 24.f, -0.5f), 1.f),
-                           t * __builtin_fmaf(t2, -1.f / 6.f, 1.f));
-    };
+    // phasors.
     float2 p[8], w[8];
-#if DD_MFMA_TRANS
-    // v_cos_f32 / v_sin_f32 take the angle in revolutions: (k0*q mod 65536)/65536
-    // and q/65536 are exact in float
-    auto lookups = [&](const u32x4& qq) {
+    // v_cos_f32 / v_sin_f32 take the angle in revolutions: (k0*q mod nfft)/nfft
+    // and q/nfft are exact in float
+    auto lookups = [&](const u32x4 (&qq)[2]) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const uint32_t q = (qq[s >> 1] >> (16 * (s & 1))) & 0xffffu;
-            const float am = (float)(__umul24(k0, q) & 0xffffu) * (1.f / 65536.f);
-            const float aw = (float)q * (1.f / 65536.f);
+            const uint32_t qs = qq[s >> 2][s & 3];
+            const float am = (float)(__umul24(k0, qs) & qmask) * qinv;
+            const float aw = (float)qs * qinv;
             p[s] = make_float2(__builtin_amdgcn_cosf(am), __builtin_amdgcn_sinf(am));
             w[s] = make_float2(__builtin_amdgcn_cosf(aw), __builtin_amdgcn_sinf(aw));
         }
     };
-#else
-    auto lookups = [&](const u32x4& qq) {
-        float2 th[8], tl[8], wh[8], wl[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t q = (qq[s >> 1] >> (16 * (s & 1))) & 0xffffu;
-            const uint32_t m8 = __umul24(8u * k0, q), w8 = q << 3;
-            th[s] = *reinterpret_cast<const float2*>(tab + ((m8 >> 8) & 0x7f8u));
-            wh[s] = *reinterpret_cast<const float2*>(tab + ((w8 >> 8) & 0x7f8u));
-            if constexpr (lopoly) {
-                tl[s] = t_lo((m8 >> 3) & 255u);
-                wl[s] = t_lo(q & 255u);
-            } else {
-                tl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (m8 & 0x7f8u));
-                wl[s] = *reinterpret_cast<const float2*>(tab + 2048 + (w8 & 0x7f8u));
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            p[s] = cmul(th[s], tl[s]);
-            w[s] = cmul(wh[s], wl[s]);
-        }
-    };
-#endif
     const bool late = DD_MFMA_STAGGER && WAVES == 8 && wave >= WAVES / 2;
 
     const uint32_t nks = (nchan + KS - 1) / KS;
     const size_t kstep = KS * rs;
     load_x();
-    u32x4 qv = *reinterpret_cast<const u32x4*>(qrow);
+    u32x4 qv[2] = {*reinterpret_cast<const u32x4*>(qrow), *reinterpret_cast<const u32x4*>(qrow + 4)};
     store_x(xs);
     if (late) lookups(qv);
     if (nks > 1) xsrc += kstep;
@@ -382,7 +323,8 @@ void fdd_mfma_k(
         // prefetch K-step min(ks + 1, nks - 1): the last iteration re-reads
         // its own K-step (in bounds, unused)
         load_x();
-        const u32x4 qn = *reinterpret_cast<const u32x4*>(qrow + min(ks + 1, nks - 1) * KS);
+        const uint32_t* qnext = qrow + min(ks + 1, nks - 1) * KS;
+        const u32x4 qn[2] = {*reinterpret_cast<const u32x4*>(qnext), *reinterpret_cast<const u32x4*>(qnext + 4)};
         xsrc += ks + 2 < nks ? kstep : 0;
 
         if (!late) lookups(qv);
@@ -546,7 +488,8 @@ void fdd_mfma_k(
 
         if (late && more) lookups(qn);
         if (more) store_x(xs + ((ks + 1) & 1) * BUF);
-        qv = qn;
+        qv[0] = qn[0];
+        qv[1] = qn[1];
         __syncthreads();
     }
 
@@ -586,14 +529,14 @@ void fdd_mfma_k(
 
 }  // namespace
 
-extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint16_t* q16,
+extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint32_t* q,
                                            size_t qpitch, size_t ndm, size_t nchan, size_t nchunk,
-                                           const float2* tab65536, size_t nfft,
+                                           size_t nfft,
                                            float xscale, float2* Y, size_t Yp, hipStream_t stream)
 {
     if (ndm == 0 || nchunk == 0) return hipSuccess;
     // layout preconditions of the vector loads and stores
-    if (qpitch % 16 || qpitch < (nchan + 15) / 16 * 16 || Yp % 2 || nfft < 2 * NB || nfft > 65536 ||
+    if (qpitch % 16 || qpitch < (nchan + 15) / 16 * 16 || Yp % 2 || nfft < 2 * NB || nfft > (size_t(1) << 18) ||
         (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
         return hipErrorInvalidValue;
     const size_t M = nfft / 2;
@@ -606,8 +549,8 @@ extern "C" hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp,
         const uint32_t nbb = (uint32_t)((M + Blk<P>::NB) / Blk<P>::NB);                             \
         const size_t nblocks = (size_t)nbb * ndb * ncg;                                             \
         if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;                              \
-        hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q16, \
-                           qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, tab65536,      \
+        hipLaunchKernelGGL(fdd_mfma_k<P>, dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q,   \
+                           qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk,                \
                            (uint32_t)nfft, xscale, yscale, Y, Yp, nbb, ndb);                        \
     }
     switch (prec) {

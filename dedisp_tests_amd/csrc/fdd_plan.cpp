@@ -68,8 +68,12 @@ struct FDDState {
         for (hipEvent_t e : blk_events) (void)hipEventDestroy(e);
     }
     DeviceBuffer d_delays;  // [ndm][nchan] int32 truncated delays
-    DeviceBuffer d_q16;     // [ndm][qpitch] uint16 phase steps d*65536/nfft mod 65536
-    DeviceBuffer d_tab;     // [512] e^{2 pi i 256a/65536}, e^{2 pi i b/65536}
+    DeviceBuffer d_q32;     // [ndm][qpitch32] uint32 phase steps d mod nfft (matrix-core forms)
+    DeviceBuffer d_dm32;    // [ndm] DM list (16 x 16 tile kernel)
+    DeviceBuffer d_delay32; // [nchan rounded up to 32] delay table, zero padded
+    DeviceBuffer d_ph64k;   // [65536] e^{+2 pi i m/65536} (split transforms, nfft > 65536)
+    DeviceBuffer d_fwork;   // forward work of the split transforms
+    DeviceBuffer d_iwork;   // inverse work of the split transforms
     DeviceBuffer d_scale;   // [nchan] per-channel input scale
     DeviceBuffer d_ph;      // [nfft] e^{+2 pi i m / nfft}
 
@@ -84,8 +88,8 @@ struct FDDState {
     unsigned long version = ~0ul;
     int32_t dmin = 0, dmax = 0;
     size_t ph_nfft = 0;
-    unsigned long q16_version = ~0ul;
-    size_t q16_nfft = 0, qpitch = 0;
+    unsigned long q32_version = ~0ul;
+    size_t q32_nfft = 0, qpitch32 = 0;
     std::vector<int32_t> h_delays;  // host copy of d_delays
     float max_scale = 0.f;
 };
@@ -102,6 +106,26 @@ bool env_flag(const char* name)
     const char* s = std::getenv(name);
     return s && std::atoi(s) != 0;
 }
+
+// The contraction kernel of a precision: the bfloat16 and float16 forms run
+// on 16 x 16 x 32 tiles with re/im along N (DEDISP_FDD_MFMA32=1 selects the
+// 32 x 32 tile kernel instead), float32 on 32 x 32 x 2 tiles
+// (DEDISP_FDD_FP32_TILE16=1: 16 x 16 x 4 tiles with re/im along N), and
+// DEDISP_FDD_VALU=1 the float32 vector-ALU form.
+enum class Contraction { VALU, MFMA32, MFMA16 };
+
+Contraction contraction_kind(dedisp_amd_fdd_precision p)
+{
+    if (p == DEDISP_AMD_FDD_FP32) {
+        if (env_flag("DEDISP_FDD_VALU")) return Contraction::VALU;
+        return env_flag("DEDISP_FDD_FP32_TILE16") ? Contraction::MFMA16 : Contraction::MFMA32;
+    }
+    return env_flag("DEDISP_FDD_MFMA32") ? Contraction::MFMA32 : Contraction::MFMA16;
+}
+
+// Channel rows of the spectra: a multiple of 32 (the K-step of the 16 x 16
+// tile kernel, a multiple of the 32 x 32 kernel's 16), extra rows zero.
+size_t padded_channels(size_t nchan) { return round_up(nchan, 32); }
 
 }  // namespace
 
@@ -174,7 +198,7 @@ double FDDGPUPlan::autotune(size_type nsamps, size_type in_nbits, size_type out_
     dedisp_size best_n = 0;
     unsigned timed = 0;
     try {
-        for (dedisp_size n = 8192; n <= 65536; n *= 2) {
+        for (dedisp_size n = 8192; n <= 262144; n *= 2) {
             if (n <= m_max_delay || !dd_fdd_fft_supported(n)) continue;
             set_segmented(1, n);
             execute(nsamps, d_in.as<uint8_t>(), in_nbits, d_out.as<uint8_t>(), 32, DEDISP_DEVICE_POINTERS);
@@ -240,6 +264,12 @@ void FDDGPUPlan::prepare_tables()
         // killed channels contribute nothing
         std::vector<float> scale(nchan);
         for (size_t c = 0; c < nchan; ++c) scale[c] = h_killmask[c] ? (float)(1.0 / nchan) : 0.f;
+        std::vector<float> dl32(round_up(nchan, 32), 0.f);
+        std::copy(h_delay_table.begin(), h_delay_table.end(), dl32.begin());
+        st.d_dm32.reserve(std::max<size_t>(ndm, 1) * 4);
+        st.d_delay32.reserve(dl32.size() * 4);
+        DD_HIP(hipMemcpyAsync(st.d_dm32.ptr, h_dm_list.data(), ndm * 4, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipMemcpyAsync(st.d_delay32.ptr, dl32.data(), dl32.size() * 4, hipMemcpyHostToDevice, stream));
         st.d_delays.reserve(d.size() * 4);
         st.d_scale.reserve(nchan * 4);
         DD_HIP(hipMemcpyAsync(st.d_delays.ptr, d.data(), d.size() * 4, hipMemcpyHostToDevice, stream));
@@ -255,8 +285,11 @@ void FDDGPUPlan::prepare_tables()
 }
 
 // Overlap-save segment length for series with nsc[0..n) outputs per DM: the
-// requested one, else the supported length with the smallest summed
-// contraction volume (chunks * (nfft/2 + 1)).
+// requested one, else the supported length (2^13 .. 2^18) with the smallest
+// summed contraction volume (chunks * (nfft/2 + 1)). The reference instead
+// doubles nfft from 16384 until a fifth of it covers the maximum delay
+// (src/fdd/FDDGPUPlan.cpp:756-762); the smallest volume needs fewer complex
+// multiply-accumulates for the same outputs.
 size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
 {
     const FDDState& st = *m_state;
@@ -265,7 +298,7 @@ size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
     if (!nfft) {
         // smallest contraction volume nchunk * (nfft/2 + 1) among the supported lengths
         size_t best = ~size_t(0);
-        for (size_t cand = 8192; cand <= 65536; cand *= 2) {
+        for (size_t cand = 8192; cand <= 262144; cand *= 2) {
             if (cand < dmax + 2 + 2) continue;
             const size_t good = (cand - dmax) & ~size_t(1);
             size_t chunks = 0;
@@ -280,9 +313,9 @@ size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
 }
 
 // Phase tables of segment length nfft: e^{2 pi i m/nfft} for the transforms
-// and, for the matrix-core contraction (valu false), the 65536-based phase
-// steps (and the two-level phasor tables its table form reads). Returns the
-// float16 spectrum scale.
+// (and the 65536-entry table of the on-chip engine when segments are split)
+// and, for the matrix-core contraction (valu false), the phase steps
+// d mod nfft. Returns the float16 spectrum scale.
 float FDDGPUPlan::prepare_phasors(size_t nfft, bool valu)
 {
     FDDState& st = *m_state;
@@ -301,31 +334,32 @@ float FDDGPUPlan::prepare_phasors(size_t nfft, bool valu)
         st.ph_nfft = nfft;
     }
 
-    // phase steps of the matrix-core contraction (65536-based phases)
-    if (!valu && (st.q16_version != m_tables_version || st.q16_nfft != nfft)) {
-        const size_t qpitch = round_up(nchan, 16);
-        const uint32_t mul = (uint32_t)(65536 / nfft);
-        std::vector<uint16_t> q(ndm * qpitch, 0);
+    // the on-chip transform engine's table when segments are split (nfft > 65536)
+    if (nfft > 65536 && !st.d_ph64k.ptr) {
+        std::vector<float> ph(2 * 65536);
+        for (size_t m = 0; m < 65536; ++m) {
+            const double a = 2.0 * M_PI * (double)m / 65536.0;
+            ph[2 * m] = (float)std::cos(a);
+            ph[2 * m + 1] = (float)std::sin(a);
+        }
+        st.d_ph64k.reserve(65536 * 8);
+        DD_HIP(hipMemcpyAsync(st.d_ph64k.ptr, ph.data(), 65536 * 8, hipMemcpyHostToDevice, stream));
+        DD_HIP(hipStreamSynchronize(stream));
+    }
+    // phase steps of the matrix-core contraction: q = d mod nfft (phases in
+    // units of 1/nfft)
+    if (!valu && (st.q32_version != m_tables_version || st.q32_nfft != nfft)) {
+        const size_t qpitch = round_up(nchan, 32);
+        std::vector<uint32_t> q(ndm * qpitch, 0);
         for (size_t i = 0; i < ndm; ++i)
             for (size_t c = 0; c < nchan; ++c)
-                q[i * qpitch + c] = (uint16_t)(((uint32_t)st.h_delays[i * nchan + c] * mul) & 0xffffu);
-        std::vector<float> tab(1024);
-        for (int a = 0; a < 256; ++a) {
-            const double t1 = 2.0 * M_PI * (double)(256 * a) / 65536.0;
-            const double t2 = 2.0 * M_PI * (double)a / 65536.0;
-            tab[2 * a] = (float)std::cos(t1);
-            tab[2 * a + 1] = (float)std::sin(t1);
-            tab[512 + 2 * a] = (float)std::cos(t2);
-            tab[512 + 2 * a + 1] = (float)std::sin(t2);
-        }
-        st.d_q16.reserve(q.size() * 2);
-        st.d_tab.reserve(tab.size() * 4);
-        DD_HIP(hipMemcpyAsync(st.d_q16.ptr, q.data(), q.size() * 2, hipMemcpyHostToDevice, stream));
-        DD_HIP(hipMemcpyAsync(st.d_tab.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, stream));
+                q[i * qpitch + c] = (uint32_t)st.h_delays[i * nchan + c] & (uint32_t)(nfft - 1);
+        st.d_q32.reserve(q.size() * 4);
+        DD_HIP(hipMemcpyAsync(st.d_q32.ptr, q.data(), q.size() * 4, hipMemcpyHostToDevice, stream));
         DD_HIP(hipStreamSynchronize(stream));
-        st.qpitch = qpitch;
-        st.q16_nfft = nfft;
-        st.q16_version = m_tables_version;
+        st.qpitch32 = qpitch;
+        st.q32_nfft = nfft;
+        st.q32_version = m_tables_version;
     }
     // float16 spectra: |X| <= nfft * 127.5 * max scale; a power of two brings
     // that bound to at most 2^15 (float16 max 65504)
@@ -336,6 +370,55 @@ float FDDGPUPlan::prepare_phasors(size_t nfft, bool valu)
     }
     return xscale;
 }
+
+namespace {
+
+// Whether the contraction writes the inverse transform's packed spectrum
+// (16 x 16 tile kernel with mirror bins; DEDISP_FDD_PACKED=0 keeps bins).
+bool env_off(const char* name)
+{
+    const char* s = std::getenv(name);
+    return s && std::atoi(s) == 0;
+}
+
+bool contraction_packs(const FDDState& st)
+{
+    return contraction_kind(st.precision) == Contraction::MFMA16 && !env_off("DEDISP_FDD_MIRROR") &&
+           !env_off("DEDISP_FDD_PACKED");
+}
+
+// Contraction of DMs b0 .. b0+nb of the plan's tables over the spectra in
+// st.d_Z ([padded channels][nchunk][Xp]) into st.d_Y ([nb][nchunk][Yp]):
+// dedispersed bins 0..M, or the packed spectrum when contraction_packs.
+void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, size_t nfft, size_t Xp, size_t Yp,
+              float xscale, hipStream_t stream)
+{
+    switch (contraction_kind(st.precision)) {
+    case Contraction::VALU:
+        DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, st.d_delays.as<int32_t>() + b0 * nchan, nb, nchan,
+                                    nchunk, st.d_ph.as<float2>(), nfft, st.d_Y.as<float2>(), Yp, stream));
+        break;
+    case Contraction::MFMA32:
+        DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Xp,
+                                    st.d_q32.as<uint32_t>() + b0 * st.qpitch32, st.qpitch32, nb, nchan, nchunk,
+                                    nfft, xscale, st.d_Y.as<float2>(), Yp, stream));
+        break;
+    case Contraction::MFMA16:
+        // DEDISP_FDD_MIRROR=0: four consecutive bins per workgroup instead of
+        // two bins and their mirrors
+        // delays from the DM list and delay table in the kernel
+        // (DEDISP_FDD_QTABLE=1: read from the phase-step table instead)
+        DD_HIP(dd_fdd_contract_mfma16((int)st.precision, !env_off("DEDISP_FDD_MIRROR"), st.d_Z.as<float2>(), Xp,
+                                      st.d_q32.as<uint32_t>() + b0 * st.qpitch32, st.qpitch32, nb, nchan, nchunk,
+                                      nfft, xscale, st.d_Y.as<float2>(), Yp,
+                                      contraction_packs(st) ? st.d_ph.as<float2>() : nullptr,
+                                      env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_dm32.as<float>() + b0,
+                                      env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_delay32.as<float>(), stream));
+        break;
+    }
+}
+
+}  // namespace
 
 void FDDGPUPlan::execute(size_type nsamps, const byte_type* in, size_type in_nbits,
                          byte_type* out, size_type out_nbits, unsigned flags)
@@ -437,8 +520,8 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     const size_t in_span = fourier ? nsamps : std::max(nsamps, (nchunk - 1) * good + nfft);
     const size_t row_words = div_up(in_span, (32 / in_nbits) * 64) * 64;
     st.d_rows.reserve(nchan * row_words * 4);
-    // channel rows padded to a multiple of 16 with zeros (matrix-core staging)
-    const size_t nchan_pad = round_up(nchan, 16);
+    // channel rows padded to a multiple of 32 with zeros (matrix-core staging)
+    const size_t nchan_pad = padded_channels(nchan);
     st.d_Z.reserve(nchan_pad * nchunk * Xp * 8);
     if (nchan_pad > nchan)
         DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nchunk * Xp, 0,
@@ -447,14 +530,15 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     DD_HIP(dd_tdd_transpose(d_in, nsamps, in_row, (unsigned)in_nbits, nchan, st.d_rows.as<uint32_t>(),
                             row_words, stream));
     ctx.timer.end(StageTimer::UNPACK, stream);
+    st.d_fwork.reserve(std::max<size_t>(dd_fdd_split_work(nfft, nchan * nchunk), 16));
     ctx.timer.begin(StageTimer::FFT, stream);
-    DD_HIP(dd_fdd_forward(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, fourier ? Nf : 0,
-                          st.d_scale.as<float>(), ph, nfft, st.d_Z.as<float2>(), nchan, nchunk, Xp,
-                          (unsigned)in_nbits, stream));
+    DD_HIP(dd_fdd_forward_split(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, fourier ? Nf : 0,
+                                st.d_scale.as<float>(), ph, st.d_ph64k.as<float2>(), nfft, st.d_Z.as<float2>(),
+                                nchan, nchunk, Xp, (unsigned)in_nbits, st.d_fwork.as<float2>(), stream));
     ctx.timer.end(StageTimer::FFT, stream);
 
     // ---- per DM batch: contraction, inverse transforms ----
-    const size_t per_dm = nchunk * Yp * 8 + (fourier ? 2 * Nf * 4 : 0);
+    const size_t per_dm = nchunk * Yp * 8 + (fourier ? 2 * Nf * 4 : 0) + dd_fdd_split_work(nfft, nchunk);
     size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
     if (!device_ptrs) {
         // host outputs: DM batches of about 1 GB of output (whole 256-DM tiles
@@ -469,6 +553,7 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
         }
     }
     st.d_Y.reserve(dm_batch * nchunk * Yp * 8);
+    st.d_iwork.reserve(std::max<size_t>(dd_fdd_split_work(nfft, dm_batch * nchunk), 16));
     if (fourier) {
         st.d_S.reserve(dm_batch * Nf * 4);
         st.d_work.reserve(dm_batch * Nf * 4);
@@ -485,23 +570,17 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
         const size_t nb = std::min(dm_batch, ndm - b0);
         ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-        const int32_t* dl = st.d_delays.as<int32_t>() + b0 * nchan;
-        if (valu)
-            DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, dl, nb, nchan, nchunk, ph, nfft,
-                                        st.d_Y.as<float2>(), Yp, stream));
-        else
-            DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Xp,
-                                        st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan,
-                                        nchunk, st.d_tab.as<float2>(), nfft, xscale,
-                                        st.d_Y.as<float2>(), Yp, stream));
+        contract(st, nchan, b0, nb, nchunk, nfft, Xp, Yp, xscale, stream);
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
         ctx.timer.begin(StageTimer::IFFT, stream);
         if (!fourier) {
-            DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, d_out + b0 * nsc, nsc,
-                                  good, nsc, stream));
+            DD_HIP(dd_fdd_inverse_split(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, st.d_ph64k.as<float2>(), nfft,
+                                        d_out + b0 * nsc, nsc, good, nsc, st.d_iwork.as<float2>(),
+                                        contraction_packs(st), stream));
         } else {
-            DD_HIP(dd_fdd_inverse(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, nfft, st.d_S.as<float>(), Nf,
-                                  good, Nf, stream));
+            DD_HIP(dd_fdd_inverse_split(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, st.d_ph64k.as<float2>(), nfft,
+                                        st.d_S.as<float>(), Nf, good, Nf, st.d_iwork.as<float2>(),
+                                        contraction_packs(st), stream));
             DD_HIP(dd_fdd_rfft_rows(st.d_S.as<float>(), Nf, nb, Nf, st.d_ph16k.as<float2>(),
                                     st.d_twL.as<float2>(), st.d_twN.as<float2>(),
                                     st.d_work.as<float2>(),
@@ -584,11 +663,12 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
     const bool valu = env_flag("DEDISP_FDD_VALU") && st.precision == DEDISP_AMD_FDD_FP32;
     const float xscale = prepare_phasors(nfft, valu);
     const float2* ph = st.d_ph.as<float2>();
-    const size_t nchan_pad = round_up(nchan, 16);
+    const size_t nchan_pad = padded_channels(nchan);
 
     // ---- layout: groups of beams, each beam's rows / staged input / output ----
     // DEDISP_FDD_GROUP_CHUNKS caps the chunks of a group below the memory bound
-    size_t max_chunks = std::max<size_t>(1, kZBudget / (nchan_pad * Xp * 8));
+    size_t max_chunks =
+        std::max<size_t>(1, kZBudget / (nchan_pad * Xp * 8 + dd_fdd_split_work(nfft, nchan)));
     if (const char* e = std::getenv("DEDISP_FDD_GROUP_CHUNKS"))
         if (std::atol(e) > 0) max_chunks = std::min(max_chunks, (size_t)std::atol(e));
     struct Group {
@@ -630,11 +710,13 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
         g_out = std::max(g_out, g.out_floats);
         g_chunks = std::max(g_chunks, g.nchunk);
     }
-    const size_t per_dm = g_chunks * Yp * 8;
+    const size_t per_dm = g_chunks * Yp * 8 + dd_fdd_split_work(nfft, g_chunks);
     const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
     st.d_rows.reserve(g_rows * 4);
     st.d_Z.reserve(nchan_pad * g_chunks * Xp * 8);
     st.d_Y.reserve(dm_batch * g_chunks * Yp * 8);
+    st.d_fwork.reserve(std::max<size_t>(dd_fdd_split_work(nfft, nchan * g_chunks), 16));
+    st.d_iwork.reserve(std::max<size_t>(dd_fdd_split_work(nfft, dm_batch * g_chunks), 16));
     if (!device_ptrs) {
         st.d_in.reserve(g_in + 64);
         st.d_out.reserve(g_out * 4);
@@ -645,7 +727,10 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
     std::vector<uint64_t> key = {nbeams, in_nbits, nfft, (uint64_t)st.precision, valu ? 1u : 0u,
                                  m_tables_version, max_chunks, dm_batch, __builtin_bit_cast(uint32_t, xscale),
                                  (uint64_t)st.d_rows.ptr, (uint64_t)st.d_Z.ptr, (uint64_t)st.d_Y.ptr,
-                                 (uint64_t)st.d_q16.ptr, (uint64_t)st.d_tab.ptr, (uint64_t)st.d_ph.ptr,
+                                 (uint64_t)st.d_q32.ptr, (uint64_t)st.d_ph.ptr, (uint64_t)st.d_ph64k.ptr,
+                                 (uint64_t)st.d_fwork.ptr, (uint64_t)st.d_iwork.ptr,
+                                 (uint64_t)contraction_kind(st.precision), (uint64_t)contraction_packs(st),
+                                 (uint64_t)!env_off("DEDISP_FDD_MIRROR"),
                                  (uint64_t)st.d_scale.ptr, (uint64_t)st.d_delays.ptr};
     for (size_t b = 0; b < nbeams; ++b) {
         key.push_back(nsamps[b]);
@@ -711,24 +796,19 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
         }
         if (timed) ctx.timer.end(StageTimer::UNPACK, stream);
         if (timed) ctx.timer.begin(StageTimer::FFT, stream);
-        DD_HIP(dd_fdd_forward_chunks(d_tab + g.chunk0, nc, st.d_scale.as<float>(), ph, nfft,
-                                     st.d_Z.as<float2>(), nchan, Xp, (unsigned)in_nbits, stream));
+        DD_HIP(dd_fdd_forward_chunks_split(d_tab + g.chunk0, nc, st.d_scale.as<float>(), ph,
+                                           st.d_ph64k.as<float2>(), nfft, st.d_Z.as<float2>(), nchan, Xp,
+                                           (unsigned)in_nbits, st.d_fwork.as<float2>(), stream));
         if (timed) ctx.timer.end(StageTimer::FFT, stream);
         for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
             const size_t nb = std::min(dm_batch, ndm - b0);
             if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-            if (valu)
-                DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, st.d_delays.as<int32_t>() + b0 * nchan, nb,
-                                            nchan, nc, ph, nfft, st.d_Y.as<float2>(), Yp, stream));
-            else
-                DD_HIP(dd_fdd_contract_mfma((int)st.precision, st.d_Z.as<float2>(), Xp,
-                                            st.d_q16.as<uint16_t>() + b0 * st.qpitch, st.qpitch, nb, nchan, nc,
-                                            st.d_tab.as<float2>(), nfft, xscale, st.d_Y.as<float2>(), Yp,
-                                            stream));
+            contract(st, nchan, b0, nb, nc, nfft, Xp, Yp, xscale, stream);
             if (timed) ctx.timer.end(StageTimer::DEDISPERSE, stream);
             if (timed) ctx.timer.begin(StageTimer::IFFT, stream);
-            DD_HIP(dd_fdd_inverse_chunks(st.d_Y.as<float2>(), nb, b0, d_tab + g.chunk0, nc, Yp, ph, nfft, good,
-                                         stream));
+            DD_HIP(dd_fdd_inverse_chunks_split(st.d_Y.as<float2>(), nb, b0, d_tab + g.chunk0, nc, Yp, ph,
+                                               st.d_ph64k.as<float2>(), nfft, good, st.d_iwork.as<float2>(),
+                                               contraction_packs(st), stream));
             if (timed) ctx.timer.end(StageTimer::IFFT, stream);
         }
     };

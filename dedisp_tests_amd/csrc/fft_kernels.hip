@@ -339,7 +339,9 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
 // its own element C-n2 (k = 0 with Y[M]), read back from a small LDS area.
 // TAB: the chunk's output row (row row0 + blockIdx.y of tab[chunk].out), first
 // output sample and output length come from tab[chunk].
-template <int A, int B, int C, bool TAB = false, int NT = FftShape<A, B, C>::T>
+// PACKED: Y already holds the packed spectrum Zu[0..M-1] (the contraction's
+// mirror form writes it), so pass 1 loads it as it is.
+template <int A, int B, int C, bool TAB = false, bool PACKED = false, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_inverse_k(
     const float2* __restrict__ Y, size_t nchunk, size_t Yp, const float2* __restrict__ ph,
     float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out,
@@ -362,8 +364,13 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     }
     float* dst = out + (row0 + row) * out_stride;
     const float inv_n = 1.0f / (float)(2 * M);
-    const float ymx = src[M].x;  // Y[M]: its imaginary part is not used
     const int t = threadIdx.x;
+    float2 q[A];
+    if constexpr (PACKED) {
+        auto zload = [&](int k) -> float2 { return src[(uint32_t)k]; };
+        fft3_core<A, B, C, +1>(ph, lds, zload, q);
+    } else {
+    const float ymx = src[M].x;  // Y[M]: its imaginary part is not used
     const int hi = (t >> 5) & 1;
     const int base = (t >> 6) * 32 + (t & 31);
     const int tp = (t < T1) ? (hi ? (base ? T1 - base : T1 / 2) : base) : t;
@@ -417,8 +424,8 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
             }
         }
     };
-    float2 q[A];
     fft3_core<A, B, C, +1>(ph, lds, load, q, pair, tp);
+    }
     if (t < S::T3) {
         // one workgroup-uniform base and limit (t0 < nsamps_out for every
         // chunk processed); per sample a 32-bit offset
@@ -466,24 +473,339 @@ hipError_t launch_forward_chunks(const dd_fdd_chunk* chunks, size_t nchunk, cons
 template <int A, int B, int C>
 hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
                           float* out, size_t out_stride, size_t good, size_t nsamps_out,
-                          hipStream_t stream)
+                          hipStream_t stream, bool packed = false)
 {
-    hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nrows),
-                       dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
-                       good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
+    if (packed)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, false, true>), dim3((unsigned)nchunk, (unsigned)nrows),
+                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
+                           good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
+    else
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nrows),
+                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
+                           good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
     return hipGetLastError();
 }
 
 template <int A, int B, int C>
 hipError_t launch_inverse_chunks(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
                                  size_t nchunk, size_t Yp, const float2* ph, size_t good,
-                                 hipStream_t stream)
+                                 hipStream_t stream, bool packed = false)
 {
     if (!chunks) return hipErrorInvalidValue;
     if (!nchunk || !nrows) return hipSuccess;
-    hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nrows),
-                       dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
-                       (size_t)0, good, (size_t)0, chunks, row0);
+    if (packed)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true, true>), dim3((unsigned)nchunk, (unsigned)nrows),
+                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
+                           (size_t)0, good, (size_t)0, chunks, row0);
+    else
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nrows),
+                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
+                           (size_t)0, good, (size_t)0, chunks, row0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Segments of nfft = 2^17 and 2^18 samples (M = nfft/2 = R*L packed points,
+// L = 32768, R = 2 or 4): one component of M points no longer fits the LDS, so
+// each transform is split in time (decimation by R) into R on-chip L-point
+// transforms plus a radix-R combine, with one round trip through a work
+// buffer of M complex values per transform.
+//
+// Forward, for n = r + R*j and k' = k + L*s (k < L, s < R):
+//   Z_r[k]   = sum_j z[r + R j] W_L^{jk}                         (split_fwd_k)
+//   Z[k+L s] = sum_r W_R^{rs} W_M^{rk} Z_r[k]                     (combine_fwd_k)
+// and the combine kernel also pairs Z into the real spectrum X[0..M]; the
+// mirror M - (k + L s) = (L - k) + L (R-1-s), so a thread taking k and L - k
+// holds every value its outputs need.
+// Inverse: Zu[k'] is formed from Y[k'] and Y[M - k'] as in fdd_inverse_k, then
+//   Zr'[k]   = W_M^{-rk} sum_s W_R^{-rs} Zu[k + L s]                (split_inv_k)
+//   z[r+R j] = sum_k Zr'[k] W_L^{-jk}                             (combine_inv_k)
+// Twiddles: ph[m] = e^{+2 pi i m/nfft}; the on-chip engine takes the table
+// of its own size, ph64k[m] = e^{+2 pi i m/65536}.
+// ---------------------------------------------------------------------------
+
+constexpr int LBIG = 32768;  // on-chip points per transform (32 x 32 x 32)
+
+// radix-R DFT of v (R = 2 or 4), DIR = -1 forward, +1 inverse
+template <int R, int DIR>
+__device__ __forceinline__ void dft_small(float2 (&v)[R])
+{
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else {
+        const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        const float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+        // DIR * i * b1
+        const float2 jb1 = (DIR < 0) ? make_float2(b1.y, -b1.x) : make_float2(-b1.y, b1.x);
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, jb1);
+        v[3] = csub(a1, jb1);
+    }
+}
+
+// split_fwd_k: residue r = blockIdx.x % R of chunk blockIdx.x / R, channel
+// blockIdx.y; writes Z_r (L points) to work[((c*nchunk + chunk)*R + r)*L].
+// TAB: the chunk's rows, first sample and valid length come from tab[chunk].
+template <int R, bool TAB = false>
+__global__ __launch_bounds__(1024) void split_fwd_k(
+    const uint8_t* __restrict__ rows, size_t row_bytes, size_t chunk_step, size_t nvalid, size_t period,
+    const float* __restrict__ chan_scale, const float2* __restrict__ ph64k, float2* __restrict__ work,
+    size_t nchunk, uint32_t nbits, const dd_fdd_chunk* __restrict__ tab)
+{
+    using S = FftShape<32, 32, 32>;
+    __shared__ float lds[S::LDS_FLOATS];
+    const size_t chunk = blockIdx.x / R, c = blockIdx.y;
+    const uint32_t r = blockIdx.x % R;
+    size_t t0 = chunk * chunk_step;
+    if constexpr (TAB) {
+        const dd_fdd_chunk e = tab[chunk];
+        rows = e.rows;
+        row_bytes = e.row_bytes;
+        nvalid = e.nvalid;
+        t0 = e.t0;
+    }
+    const uint8_t* src = rows + c * row_bytes;
+    const float s = chan_scale[c];
+    const float mid = 0.5f * (float)((1u << nbits) - 1u);
+    const uint32_t lspw = 5 - __builtin_ctz(nbits);
+    const uint32_t mask = (1u << nbits) - 1u;
+    const uint32_t* wsrc = reinterpret_cast<const uint32_t*>(src);
+    auto value = [&](size_t t) -> uint32_t {
+        if (nbits == 8) return src[t];
+        return (wsrc[t >> lspw] >> ((uint32_t)(t & ((1u << lspw) - 1u)) * nbits)) & mask;
+    };
+    auto sample = [&](size_t t) -> float {
+        if (period) t %= period;
+        return (t < nvalid) ? ((float)value(t) - mid) * s : 0.f;
+    };
+    // packed point r + R*n: samples t0 + 2(r + R n) and the next
+    auto load = [&](int n) -> float2 {
+        const size_t t = t0 + 2 * ((size_t)r + (size_t)R * (size_t)n);
+        if (!period && nbits == 8 && t + 1 < nvalid) {
+            const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t);
+            return make_float2(((float)(pair & 0xff) - mid) * s, ((float)(pair >> 8) - mid) * s);
+        }
+        return make_float2(sample(t), sample(t + 1));
+    };
+    float2 q[32];
+    fft3_core<32, 32, 32, -1>(ph64k, lds, load, q);
+    float2* dst = work + ((c * nchunk + chunk) * R + r) * (size_t)LBIG;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k0 = 0; k0 < 32; ++k0) dst[t + S::T3 * k0] = q[k0];
+}
+
+// combine_fwd_k: transform (c, chunk) = blockIdx.x; threads take k = 0 ..
+// L/2 (and L - k); writes X[0..M] of the transform at X + blockIdx.x * Xp.
+template <int R>
+__global__ __launch_bounds__(256) void combine_fwd_k(const float2* __restrict__ work, const float2* __restrict__ ph,
+                                                     uint32_t nfft, float2* __restrict__ X, size_t Xp)
+{
+    constexpr uint32_t L = LBIG, M = R * LBIG;
+    const float2* zr = work + (size_t)blockIdx.x * M;
+    float2* dst = X + (size_t)blockIdx.x * Xp;
+    const uint32_t mask = nfft - 1;
+    // W_M^{rk} = conj(ph[2 r k])
+    auto zk = [&](uint32_t k, float2 (&z)[R]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 v = zr[(size_t)r * L + (k & (L - 1))];
+            z[r] = r ? cmul(v, conjf2(ph[(2u * r * k) & mask])) : v;
+        }
+        dft_small<R, -1>(z);  // z[s] = Z[k + L s]
+    };
+    // X[kk] from Z[kk] = a and Z[M - kk] = m
+    auto pair_out = [&](uint32_t kk, float2 a, float2 m) {
+        const float2 pk = ph[kk];
+        const float ux = a.x + m.x, uy = a.y - m.y;
+        const float vx = a.x - m.x, vy = a.y + m.y;
+        const float wr = vx * pk.x + vy * pk.y, wi = vy * pk.x - vx * pk.y;
+        dst[kk] = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
+    };
+    for (uint32_t k = threadIdx.x; k <= L / 2; k += 256) {
+        float2 a[R];
+        zk(k, a);
+        if (k == 0) {
+            // Z[L s] pairs with Z[M - L s] = Z[L (R - s)] (Z[M] = Z[0])
+#pragma unroll
+            for (int s = 0; s < R; ++s) pair_out((uint32_t)s * L, a[s], a[(R - s) % R]);
+            dst[M] = make_float2(a[0].x - a[0].y, 0.f);
+        } else if (k == L / 2) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) pair_out(k + (uint32_t)s * L, a[s], a[R - 1 - s]);
+        } else {
+            float2 b[R];
+            zk(L - k, b);
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                pair_out(k + (uint32_t)s * L, a[s], b[R - 1 - s]);
+                pair_out(L - k + (uint32_t)s * L, b[s], a[R - 1 - s]);
+            }
+        }
+    }
+}
+
+// split_inv_k: row / chunk transform blockIdx.x of Y (bins 0..M at Yp); writes
+// Zr' (R x L points) to work[blockIdx.x*M].
+template <int R, bool PACKED = false>
+__global__ __launch_bounds__(256) void split_inv_k(const float2* __restrict__ Y, size_t Yp,
+                                                   const float2* __restrict__ ph, uint32_t nfft,
+                                                   float2* __restrict__ work)
+{
+    constexpr uint32_t L = LBIG, M = R * LBIG;
+    const float2* src = Y + (size_t)blockIdx.x * Yp;
+    float2* dst = work + (size_t)blockIdx.x * M;
+    const uint32_t mask = nfft - 1;
+    if constexpr (PACKED) {
+        // Y holds Zu[0..M-1]: the radix-R split only
+        for (uint32_t k = threadIdx.x; k < L; k += 256) {
+            float2 u[R];
+#pragma unroll
+            for (int s = 0; s < R; ++s) u[s] = src[k + (uint32_t)s * L];
+            dft_small<R, +1>(u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[(size_t)r * L + k] = r ? cmul(u[r], ph[(2u * r * k) & mask]) : u[r];
+        }
+        return;
+    }
+    const float ymx = src[M].x;
+    // Zu[k'] = (Y[k'] + conj Y[M-k']) + i e^{+2 pi i k'/nfft} (Y[k'] - conj Y[M-k'])
+    auto zu = [&](uint32_t kk, float2 a, float2 m) -> float2 {
+        float2 b = conjf2(m);
+        if (kk == 0) a.y = 0.f, b = make_float2(ymx, 0.f);
+        const float2 e = cadd(a, b), d = csub(a, b);
+        const float2 o = cmul(d, ph[kk]);
+        return make_float2(e.x - o.y, e.y + o.x);
+    };
+    // Zr'[k] = W_M^{-rk} sum_s W_R^{-rs} Zu[k + L s]
+    auto out = [&](uint32_t k, float2 (&u)[R]) {
+        dft_small<R, +1>(u);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[(size_t)r * L + k] = r ? cmul(u[r], ph[(2u * r * k) & mask]) : u[r];
+    };
+    for (uint32_t k = threadIdx.x; k <= L / 2; k += 256) {
+        float2 ya[R], u[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) ya[s] = src[k + (uint32_t)s * L];
+        if (k == 0) {
+            // Y[M - L s] = Y[L (R - s)] for s > 0; Y[M] for s = 0 (taken by zu)
+#pragma unroll
+            for (int s = 0; s < R; ++s) u[s] = zu((uint32_t)s * L, ya[s], ya[(R - s) % R]);
+            out(0, u);
+        } else if (k == L / 2) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) u[s] = zu(k + (uint32_t)s * L, ya[s], ya[R - 1 - s]);
+            out(k, u);
+        } else {
+            float2 yb[R], w[R];
+#pragma unroll
+            for (int s = 0; s < R; ++s) yb[s] = src[L - k + (uint32_t)s * L];
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                u[s] = zu(k + (uint32_t)s * L, ya[s], yb[R - 1 - s]);
+                w[s] = zu(L - k + (uint32_t)s * L, yb[s], ya[R - 1 - s]);
+            }
+            out(k, u);
+            out(L - k, w);
+        }
+    }
+}
+
+// combine_inv_k: residue r = blockIdx.x % R of chunk blockIdx.x / R, row
+// blockIdx.y: inverse L-point transform of Zr', samples 2(r + R j) and the
+// next of the chunk, times 1/nfft, the first `good` of them written. TAB: the
+// chunk's output row (row row0 + blockIdx.y of tab[chunk].out), first output
+// sample and output length come from tab[chunk].
+template <int R, bool TAB = false>
+__global__ __launch_bounds__(1024) void combine_inv_k(const float2* __restrict__ work, size_t nchunk,
+                                                      const float2* __restrict__ ph64k, float* __restrict__ out,
+                                                      size_t out_stride, size_t good, size_t nsamps_out,
+                                                      const dd_fdd_chunk* __restrict__ tab, size_t row0)
+{
+    using S = FftShape<32, 32, 32>;
+    __shared__ float lds[S::LDS_FLOATS];
+    const size_t chunk = blockIdx.x / R, row = blockIdx.y;
+    const uint32_t r = blockIdx.x % R;
+    const float2* src = work + ((row * nchunk + chunk) * R + r) * (size_t)LBIG;
+    size_t t0 = chunk * good;
+    if constexpr (TAB) {
+        const dd_fdd_chunk e = tab[chunk];
+        out = e.out;
+        out_stride = e.out_stride;
+        nsamps_out = e.nsamps_out;
+        t0 = e.t0;
+    }
+    auto load = [&](int k) -> float2 { return src[(uint32_t)k]; };
+    float2 q[32];
+    fft3_core<32, 32, 32, +1>(ph64k, lds, load, q);
+    float* const base = out + (row0 + row) * out_stride + t0;
+    const uint32_t lim = (uint32_t)min(good, nsamps_out - t0);
+    const float inv_n = 1.0f / (float)(2 * R * LBIG);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k0 = 0; k0 < 32; ++k0) {
+        const uint32_t ta = 2u * (r + (uint32_t)R * ((uint32_t)t + (uint32_t)S::T3 * k0));
+        if (ta < lim) base[ta] = q[k0].x * inv_n;
+        if (ta + 1 < lim) base[ta + 1] = q[k0].y * inv_n;
+    }
+}
+
+template <int R>
+hipError_t launch_forward_split(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
+                                size_t period, const float* chan_scale, const float2* ph, const float2* ph64k,
+                                float2* X, size_t nchan, size_t nchunk, size_t Xp, unsigned nbits, float2* work,
+                                hipStream_t stream, const dd_fdd_chunk* tab = nullptr)
+{
+    constexpr size_t M = (size_t)R * LBIG;
+    if (Xp < M + 1 || !work || !ph64k) return hipErrorInvalidValue;
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return hipErrorInvalidValue;
+    if (nbits != 8 && (row_bytes % 4)) return hipErrorInvalidValue;
+    if (!nchunk || !nchan) return hipSuccess;
+    if (nchunk * R >= (size_t(1) << 31) || nchan * nchunk >= (size_t(1) << 31)) return hipErrorInvalidValue;
+    if (tab)
+        hipLaunchKernelGGL((split_fwd_k<R, true>), dim3((unsigned)(nchunk * R), (unsigned)nchan), dim3(1024), 0,
+                           stream, (const uint8_t*)nullptr, (size_t)0, (size_t)0, (size_t)0, (size_t)0, chan_scale,
+                           ph64k, work, nchunk, (uint32_t)nbits, tab);
+    else
+        hipLaunchKernelGGL((split_fwd_k<R>), dim3((unsigned)(nchunk * R), (unsigned)nchan), dim3(1024), 0, stream,
+                           rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph64k, work, nchunk,
+                           (uint32_t)nbits, (const dd_fdd_chunk*)nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // X rows of transform (c, chunk) at (c*nchunk + chunk)*Xp: one row per blockIdx.y
+    hipLaunchKernelGGL((combine_fwd_k<R>), dim3((unsigned)(nchan * nchunk)), dim3(256), 0, stream, work, ph,
+                       (uint32_t)(2 * M), X, Xp);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_inverse_split(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
+                                const float2* ph64k, float* out, size_t out_stride, size_t good, size_t nsamps_out,
+                                float2* work, hipStream_t stream, const dd_fdd_chunk* tab = nullptr,
+                                size_t row0 = 0, bool packed = false)
+{
+    constexpr size_t M = (size_t)R * LBIG;
+    if (Yp < M + 1 || !work || !ph64k) return hipErrorInvalidValue;
+    if (!nchunk || !nrows) return hipSuccess;
+    if (nrows * nchunk >= (size_t(1) << 31) || nchunk * R >= (size_t(1) << 31)) return hipErrorInvalidValue;
+    if (packed)
+        hipLaunchKernelGGL((split_inv_k<R, true>), dim3((unsigned)(nrows * nchunk)), dim3(256), 0, stream, Y, Yp,
+                           ph, (uint32_t)(2 * M), work);
+    else
+        hipLaunchKernelGGL((split_inv_k<R>), dim3((unsigned)(nrows * nchunk)), dim3(256), 0, stream, Y, Yp, ph,
+                           (uint32_t)(2 * M), work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (tab)
+        hipLaunchKernelGGL((combine_inv_k<R, true>), dim3((unsigned)(nchunk * R), (unsigned)nrows), dim3(1024), 0,
+                           stream, work, nchunk, ph64k, (float*)nullptr, (size_t)0, good, (size_t)0, tab, row0);
+    else
+        hipLaunchKernelGGL((combine_inv_k<R>), dim3((unsigned)(nchunk * R), (unsigned)nrows), dim3(1024), 0, stream,
+                           work, nchunk, ph64k, out, out_stride, good, nsamps_out, (const dd_fdd_chunk*)nullptr,
+                           (size_t)0);
     return hipGetLastError();
 }
 
@@ -697,7 +1019,96 @@ extern "C" hipError_t dd_zero_fill(void* p, size_t bytes, hipStream_t stream)
 
 extern "C" int dd_fdd_fft_supported(size_t nfft)
 {
-    return nfft == 8192 || nfft == 16384 || nfft == 32768 || nfft == 65536;
+    return nfft == 8192 || nfft == 16384 || nfft == 32768 || nfft == 65536 || nfft == 131072 || nfft == 262144;
+}
+
+extern "C" size_t dd_fdd_split_work(size_t nfft, size_t ntransforms)
+{
+    return nfft > 65536 ? ntransforms * (nfft / 2) * sizeof(float2) : 0;
+}
+
+extern "C" hipError_t dd_fdd_forward_split(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
+                                           size_t period, const float* chan_scale, const float2* ph,
+                                           const float2* ph64k, size_t nfft, float2* X, size_t nchan,
+                                           size_t nchunk, size_t Xp, unsigned nbits, float2* work,
+                                           hipStream_t stream)
+{
+    switch (nfft) {
+    case 131072:
+        return launch_forward_split<2>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, ph64k, X, nchan,
+                                       nchunk, Xp, nbits, work, stream);
+    case 262144:
+        return launch_forward_split<4>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, ph64k, X, nchan,
+                                       nchunk, Xp, nbits, work, stream);
+    default:
+        return dd_fdd_forward(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, nfft, X, nchan, nchunk,
+                              Xp, nbits, stream);
+    }
+}
+
+extern "C" hipError_t dd_fdd_forward_chunks_split(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
+                                                  const float2* ph, const float2* ph64k, size_t nfft, float2* X,
+                                                  size_t nchan, size_t Xp, unsigned nbits, float2* work,
+                                                  hipStream_t stream)
+{
+    if (!chunks) return hipErrorInvalidValue;
+    switch (nfft) {
+    case 131072:
+        return launch_forward_split<2>(nullptr, 0, 0, 0, 0, chan_scale, ph, ph64k, X, nchan, nchunk, Xp, nbits, work,
+                                       stream, chunks);
+    case 262144:
+        return launch_forward_split<4>(nullptr, 0, 0, 0, 0, chan_scale, ph, ph64k, X, nchan, nchunk, Xp, nbits, work,
+                                       stream, chunks);
+    default:
+        return dd_fdd_forward_chunks(chunks, nchunk, chan_scale, ph, nfft, X, nchan, Xp, nbits, stream);
+    }
+}
+
+extern "C" hipError_t dd_fdd_inverse_chunks_split(const float2* Y, size_t nrows, size_t row0,
+                                                  const dd_fdd_chunk* chunks, size_t nchunk, size_t Yp,
+                                                  const float2* ph, const float2* ph64k, size_t nfft, size_t good,
+                                                  float2* work, int packed, hipStream_t stream)
+{
+    if (!chunks) return hipErrorInvalidValue;
+    switch (nfft) {
+    case 8192:  return launch_inverse_chunks<16, 16, 16>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
+    case 16384: return launch_inverse_chunks<16, 16, 32>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
+    case 32768: return launch_inverse_chunks<16, 32, 32>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
+    case 65536: return launch_inverse_chunks<32, 32, 32>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
+    case 131072:
+        return launch_inverse_split<2>(Y, nrows, nchunk, Yp, ph, ph64k, nullptr, 0, good, 0, work, stream, chunks,
+                                       row0, packed);
+    case 262144:
+        return launch_inverse_split<4>(Y, nrows, nchunk, Yp, ph, ph64k, nullptr, 0, good, 0, work, stream, chunks,
+                                       row0, packed);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+extern "C" hipError_t dd_fdd_inverse_split(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
+                                           const float2* ph64k, size_t nfft, float* out, size_t out_stride,
+                                           size_t good, size_t nsamps_out, float2* work, int packed,
+                                           hipStream_t stream)
+{
+    switch (nfft) {
+    case 8192:
+        return launch_inverse<16, 16, 16>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream, packed);
+    case 16384:
+        return launch_inverse<16, 16, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream, packed);
+    case 32768:
+        return launch_inverse<16, 32, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream, packed);
+    case 65536:
+        return launch_inverse<32, 32, 32>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream, packed);
+    case 131072:
+        return launch_inverse_split<2>(Y, nrows, nchunk, Yp, ph, ph64k, out, out_stride, good, nsamps_out, work,
+                                       stream, nullptr, 0, packed);
+    case 262144:
+        return launch_inverse_split<4>(Y, nrows, nchunk, Yp, ph, ph64k, out, out_stride, good, nsamps_out, work,
+                                       stream, nullptr, 0, packed);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 extern "C" hipError_t dd_fdd_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step,

@@ -271,3 +271,116 @@ def test_fdd_host_pointers_many_dm_batches(dd):
     assert plan.execute_raw(nsamps, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
     torch.cuda.synchronize()
     assert np.array_equal(out, d_out.cpu().numpy())
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32-tile16"])
+@pytest.mark.parametrize("nchunk_target", [1, 8, 9, 16, 23, 25, 33])
+def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nchunk_target):
+    """The 16 x 16 tile contraction (default for bfloat16 / float16) with
+    mirror bins writing the inverse's packed spectrum (default) or plain bins,
+    without mirror bins, and the 32 x 32 tile kernel, against the oracle for
+    chunk counts that fill 1, 2 or 3 N-tiles and several chunk groups; 100
+    channels pad to 128 rows."""
+    p = {"bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16, "fp32-tile16": dd.FDD_FP32}[prec]
+    if prec == "fp32-tile16":
+        monkeypatch.setenv("DEDISP_FDD_FP32_TILE16", "1")
+    nch, dt, f0, df = 100, 64e-6, 1581.8, -1.0
+    dms = np.linspace(0, 150, 70, dtype=f32)
+    probe = _fdd_plan(dd, nch, dt, f0, df, nfft=8192)
+    probe.set_dm_list(dms)
+    good = (8192 - probe.max_delay) & ~1
+    nsamps = probe.max_delay + nchunk_target * good - good // 3
+    probe.close()
+    rng = np.random.default_rng(nchunk_target)
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    ref = None
+    outs = {}
+    for mode, env in (("mirror", {}), ("mirror-bins", {"DEDISP_FDD_PACKED": "0"}),
+                      ("direct", {"DEDISP_FDD_MIRROR": "0"}), ("qtable", {"DEDISP_FDD_QTABLE": "1"}),
+                      ("tile32", {"DEDISP_FDD_MFMA32": "1"})):
+        for k in ("DEDISP_FDD_MIRROR", "DEDISP_FDD_MFMA32", "DEDISP_FDD_PACKED", "DEDISP_FDD_QTABLE"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        plan = _fdd_plan(dd, nch, dt, f0, df, nfft=8192, prec=p)
+        plan.set_dm_list(dms)
+        outs[mode] = plan.execute(data, 8, 32)
+        if ref is None:
+            ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+        plan.close()
+    tol = PREC_TOL["fp32" if prec == "fp32-tile16" else prec]
+    errs = {mode: rel_l2(out, ref) for mode, out in outs.items()}
+    assert all(e < tol for e in errs.values()), errs
+
+
+# ---------------------------------------------------------------------------
+# Maximum delays beyond 2^16 samples: split segments of 2^17 and 2^18
+# ---------------------------------------------------------------------------
+
+LONG_DELAY = dict(nch=128, dt=64e-6, f0=400.0, df=-1.0)   # 400 -> 273 MHz: ~465 samples per unit DM
+
+
+def _long_delay_case(orc, nsamps, seed):
+    g = LONG_DELAY
+    delays = orc.delay_table(g["nch"], f32(g["dt"]), f32(g["f0"]), f32(g["df"]))
+    dms = np.linspace(0.0, 172.0, 24, dtype=f32)
+    rng = np.random.default_rng(seed)
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, g["nch"]))), 0, 255).astype(np.uint8)
+    return delays, dms, data
+
+
+@pytest.mark.parametrize("nfft,prec", [(0, "fp32"), (131072, "fp32"), (262144, "fp32"), (131072, "bf16"),
+                                       (262144, "fp16")])
+def test_fdd_long_delay_timeseries_vs_oracle(dd, orc, nfft, prec):
+    """max_delay ~ 80000 samples: segments of 2^17 / 2^18 (split transforms),
+    the automatic choice included, against the oracle."""
+    g = LONG_DELAY
+    nsamps = 230000
+    delays, dms, data = _long_delay_case(orc, nsamps, 5)
+    p = {"fp32": dd.FDD_FP32, "bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec]
+    plan = _fdd_plan(dd, g["nch"], g["dt"], g["f0"], g["df"], nfft=nfft, prec=p)
+    if nfft == 0:
+        plan.fdd_set_segmented(True, 0)
+    plan.set_dm_list(dms)
+    assert 78000 < plan.max_delay < 82000
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_timeseries(g["nch"], delays, dms, nsamps, plan.max_delay, data)
+    assert out.shape == ref.shape
+    err = rel_l2(out, ref)
+    assert err < PREC_TOL[prec], err
+    plan.close()
+
+
+def test_fdd_long_delay_fourier_vs_oracle(dd, orc):
+    """The default Fourier-coefficient output with max_delay ~ 80000 (the
+    circular shift-sum comes from split segments)."""
+    g = LONG_DELAY
+    nsamps = 150000
+    delays, dms, data = _long_delay_case(orc, nsamps, 6)
+    plan = dd.Plan(g["nch"], g["dt"], g["f0"], g["df"], impl=dd.FDD)
+    plan.set_dm_list(dms)
+    N = plan.out_length(nsamps)
+    assert N == ((nsamps + plan.max_delay + 16383) // 16384) * 16384
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_fourier(g["nch"], delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(out, ref) < REL_FP32
+    plan.close()
+
+
+@pytest.mark.parametrize("nfft", [131072, 262144])
+def test_fdd_long_delay_batch_split_segments(dd, orc, nfft):
+    """Batched execution with split segments (the chunks of both beams through
+    one forward, contraction and inverse sequence): each beam equals a single
+    execute at the same segment length, and the oracle."""
+    g = LONG_DELAY
+    plan = _fdd_plan(dd, g["nch"], g["dt"], g["f0"], g["df"], nfft=nfft, prec=dd.FDD_BF16)
+    delays, dms, data = _long_delay_case(orc, 200000, 8)
+    plan.set_dm_list(dms)
+    beams = [np.ascontiguousarray(data[:120000]), np.ascontiguousarray(data[:200000])]
+    outs = plan.execute_batch(beams, 8, 32)
+    for b, o in zip(beams, outs):
+        assert np.array_equal(o, plan.execute(b, 8, 32))
+        ref = orc.fdd_timeseries(g["nch"], delays, dms, b.shape[0], plan.max_delay, b)
+        assert rel_l2(o, ref) < PREC_TOL["bf16"]
+    plan.close()
