@@ -21,7 +21,8 @@ Prints one JSON line. Besides the contract fields it carries:
                  against the int32 VALU lane rate; the kernel packs 4 sums per
                  32-bit lane, so the fraction can exceed 1)
   batch          64 beams of uneven length through one TDD graph replay, and
-                 FDD-BF16 on 8 of them with their chunks concatenated
+                 FDD-BF16 on all 64 with their chunks concatenated (one graph),
+                 with its error against the FP64 oracle
   stream         the streaming front end on 32 beams x 1 s chunks, 1024 ch,
                  3000 DMs: push latency percentiles and real-time factor, per
                  beam (TDD) and with one batched push per round (TDD,
@@ -35,8 +36,9 @@ Prints one JSON line. Besides the contract fields it carries:
   cpu_baseline   the reference's CPU FDD plan (restated as written, OpenMP)
                  timed on this host on a bounded DM sample, scaled to the
                  full DM list
-  parity         DM 0 row checked bit-for-bit against an independent GPU
-                 channel sum
+  parity         DM 0 checked bit for bit against an independent channel
+                 sum and 9 spread DM rows (the largest DM included) against the
+                 oracle; the bench exits non-zero on a mismatch
 """
 from __future__ import annotations
 
@@ -114,9 +116,9 @@ def make_input(torch, seed: int = 1234):
 
 def cpu_baseline(seconds_budget: float = 30.0):
     """Reference CPU FDD plan restated as written (oracle, OpenMP over DMs) on the
-    same geometry with 8-bit input (the CPU plan reads 8-bit only), a sample of
-    DMs; the DM-independent transforms are timed once and the per-DM part is
-    scaled to the full list."""
+    FDD leg's workload (1024 ch x 2^20 8-bit samples, 2048 trials 0..1023.5), a
+    sample of DMs; the DM-independent transforms are timed once and the per-DM
+    part is scaled to the full list. The host's FFTW is used when it loads."""
     import oracle
     ncores = os.cpu_count() or 1
     try:
@@ -125,7 +127,8 @@ def cpu_baseline(seconds_budget: float = 30.0):
         pass
     threads = int(os.environ.get("OMP_NUM_THREADS", ncores))
     delays = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
-    dms = oracle.dm_list(0.0, 1000.0, DT, 40.0, F0, DF, NCH, 1.25)
+    # the DM list of the FDD leg: 2048 trials 0 .. 1023.5
+    dms = (np.arange(2048, dtype=np.float32) * np.float32(0.5)).astype(np.float32)
     maxd = oracle.max_delay(dms[-1], delays[-1])
     nsc = NSAMPS - maxd
     rng = np.random.default_rng(5)
@@ -138,6 +141,9 @@ def cpu_baseline(seconds_budget: float = 30.0):
     wall = time.perf_counter() - t0
     per_dm = (times[1] + times[2]) / sample
     full = times[0] + per_dm * len(dms)
+    backend = oracle.fdd_cpu_fft_backend()
+    fft_note = ("the host's FFTW (libfftw3f.so.3)" if backend == "fftw3f" else
+                "own mixed-radix FFT (libfftw3f.so.3 not found on this host)")
     return {
         "value": len(dms) * nsc / full,
         "unit": "DM*samples/s",
@@ -146,7 +152,8 @@ def cpu_baseline(seconds_budget: float = 30.0):
         "sample": (f"FDDCPUPlan::execute_cpu restated (oracle/fdd_cpu.c), 1024 ch x 2^20 8-bit "
                    f"samples, {sample} of {len(dms)} DMs timed ({wall:.1f} s wall): forward "
                    f"transforms {times[0]:.2f} s once + {per_dm:.2f} s per DM, scaled to "
-                   f"{len(dms)} DMs; own mixed-radix FFT in place of FFTW"),
+                   f"the {len(dms)} DMs of the FDD leg (0..1023.5); transforms: {fft_note}"),
+        "fft": backend,
     }
 
 
@@ -317,10 +324,10 @@ def batch_leg(torch, dd, steps: int = 2) -> dict:
     samples clip(round(N(1.5, 0.8)), 0, 3), 1024 linear DM trials 0..2000. TDD
     with the autotuned tiling: one transpose per beam and one channel-sum
     launch for all beams, replayed as one HIP graph (device pointers, 8-bit
-    output); then FDD-BF16 on the first 8 beams (the overlap-save chunks of the
-    8 beams concatenated into one forward transform, contraction and inverse
-    transform sequence, replayed as one HIP graph; float output). Inputs and
-    outputs stay in HBM."""
+    output); then FDD-BF16 on all 64 beams (the overlap-save chunks of the
+    beams concatenated into forward transform, contraction and inverse
+    transform launches, replayed as one HIP graph; float output, 134 GB).
+    Inputs and outputs stay in HBM."""
     nb, nch, dt, f0, df = 64, 512, 81.92e-6, 1500.0, -0.78125
     rng = np.random.default_rng(4)
     lengths = [int(2 ** rng.uniform(17, 20)) for _ in range(nb)]
@@ -366,35 +373,40 @@ def batch_leg(torch, dd, steps: int = 2) -> dict:
     del d_out, one, plan
     torch.cuda.empty_cache()
 
-    # FDD-BF16 on the first 8 beams, chunks of all 8 in one sequence
-    sub = list(range(8))
+    # FDD-BF16 on all 64 beams: the overlap-save chunks of every beam go
+    # through one forward transform, contraction and inverse transform per
+    # group of beams, captured as one HIP graph; error against the oracle on
+    # sampled rows of the shortest and the longest beam
+    import oracle
     plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
     plan.fdd_set_output(dd.FDD_TIMESERIES)
     plan.fdd_set_precision(dd.FDD_BF16)
     plan.set_dm_list(dms)
     ftune = plan.autotune(1 << 20, 2, 32)
-    f_out = [torch.empty((1024, lengths[b] - md), dtype=torch.float32, device="cuda") for b in sub]
-    rc = plan.execute_batch_raw([lengths[b] for b in sub], [d_in[b] for b in sub], 2, f_out, 32,
-                                dd.DEVICE_POINTERS)
+    f_out = [torch.empty((1024, n - md), dtype=torch.float32, device="cuda") for n in lengths]
+    rc = plan.execute_batch_raw(lengths, d_in, 2, f_out, 32, dd.DEVICE_POINTERS)
     assert rc == 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        assert plan.execute_batch_raw([lengths[b] for b in sub], [d_in[b] for b in sub], 2, f_out, 32,
-                                      dd.DEVICE_POINTERS) == 0
+        assert plan.execute_batch_raw(lengths, d_in, 2, f_out, 32, dd.DEVICE_POINTERS) == 0
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / steps
-    # parity spot check: the shortest of the 8 beams equals its single execute
-    i = int(np.argmin([lengths[b] for b in sub]))
-    one = torch.empty_like(f_out[i])
-    assert plan.execute_raw(lengths[i], d_in[i], 2, one, 32, dd.DEVICE_POINTERS) == 0
-    torch.cuda.synchronize()
-    fparity = "ok" if torch.equal(one, f_out[i]) else "MISMATCH"
-    del one
-    out["fdd_bf16_8beams"] = {"value": 1024 * sum(lengths[b] - md for b in sub) / el, "unit": "DM*samples/s",
-                              "ms_per_batch": 1e3 * el, "nfft": ftune["nfft"],
-                              "launch": "one HIP graph: 8 transposes + 1 forward, 1 contraction, 1 inverse launch per DM batch",
-                              "parity": fparity}
+    otab = oracle.delay_table(nch, np.float32(dt), np.float32(f0), np.float32(df))
+    rows = [0, 511, 1023]
+    errs = {}
+    for b in (int(np.argmin(lengths)), int(np.argmax(lengths))):
+        host = dd.unpack_samples(d_in[b].cpu().numpy(), 2, nch)
+        ref = oracle.fdd_timeseries(nch, otab, dms[rows], lengths[b], md, host, nbits=2)
+        got = f_out[b][rows].cpu().numpy().astype(np.float64)
+        errs[f"beam{b}_{lengths[b]}"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        del host
+    out["fdd_bf16"] = {"value": 1024 * sum(n - md for n in lengths) / el, "unit": "DM*samples/s",
+                       "ms_per_batch": 1e3 * el, "nfft": ftune["nfft"], "beams": nb,
+                       "launch": "one HIP graph: 64 transposes + per group of beams 1 forward, and per DM batch "
+                                 "1 contraction and 1 inverse launch",
+                       "rel_l2_vs_fp64": errs,
+                       "rel_l2_note": "rows 0, 511, 1023 of the shortest and the longest beam against the oracle"}
     del f_out, plan, d_in
     torch.cuda.empty_cache()
     return out
@@ -721,13 +733,24 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = ndm * nsc / (elapsed / args.steps)
 
-    # parity spot check: DM 0 is the plain channel sum (independent torch sum)
+    # parity: DM 0 is the plain channel sum (independent torch sum), and 9
+    # spread DM rows, the largest DM included, equal the oracle bit for bit
     row0 = d_out[0].cpu().numpy()
     sums = s[:nsc].to(torch.int32).sum(dim=1).cpu().numpy().astype(np.float32)
     f32 = np.float32
     factor = (f32(3.0) * f32(1024.0)) / f32(255.0) / f32(16.0)
     ref0 = ((sums * f32(1.0)) / (f32(15.0) * f32(NCH)) * factor).astype(f32)
-    parity = "ok" if np.array_equal(row0, ref0) else "MISMATCH"
+    import oracle
+    parity_rows = sorted(set(np.linspace(0, ndm - 1, 9).astype(int).tolist()))
+    host_packed = d_in.cpu().numpy()
+    otab = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
+    ref_rows = oracle.tdd(NCH, otab, None, plan.dm_list()[parity_rows], NSAMPS, maxd, host_packed, NBITS, 32)
+    got_rows = d_out[parity_rows].cpu().numpy()
+    bad = [r for j, r in enumerate(parity_rows) if not np.array_equal(got_rows[j], ref_rows[j])]
+    parity = "ok" if (np.array_equal(row0, ref0) and not bad) else "MISMATCH"
+    parity_detail = {"rows_checked": len(parity_rows) + 1, "rows": [0] + parity_rows, "mismatched_rows": bad,
+                     "against": "oracle TDD (bit for bit) and an independent channel sum for DM 0"}
+    del host_packed, got_rows, ref_rows
 
     k_ms = float(np.mean(kernel_ms))
     in_bytes = NSAMPS * NCH * NBITS / 8
@@ -761,8 +784,13 @@ def main():
         "valu_roofline": valu_roofline(k_ms, adds_t),
         "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
         "parity": parity,
+        "parity_detail": parity_detail,
     }
     progress("tdd", t_start)
+    if parity != "ok":
+        print(json.dumps(line))
+        print(f"bench: TDD output differs from the oracle on rows {bad}", file=sys.stderr)
+        sys.exit(1)
     peaks = peaks_leg(torch)
     line["measured_peaks"] = peaks
     line["roofline"]["frac_of_measured_peak"] = achieved_gbs / peaks["hbm_copy_gbs"]

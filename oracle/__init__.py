@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
                                           C.c_ulong, C.c_ulong, C.c_ulong, C.c_void_p, C.c_void_p,
                                           C.c_void_p]
         l.oracle_fdd_cpu_asis.restype = C.c_int
+        l.oracle_fdd_cpu_fft_backend.argtypes = []
+        l.oracle_fdd_cpu_fft_backend.restype = C.c_char_p
         for suf, ct in (("d", C.c_double), ("f", C.c_float)):
             for name in (f"oracle_rfft_{suf}", f"oracle_irfft_{suf}"):
                 getattr(l, name).argtypes = [C.c_ulong, C.c_void_p, C.c_void_p]
@@ -205,6 +207,12 @@ def fdd_cpu_asis(nchans, dt, delays, dms, nsamps, max_delay_, data_u8, dm_first=
                                    _p(data_u8), _p(out), None if times is None else _p(times))
     assert rc == 0
     return out
+
+
+def fdd_cpu_fft_backend() -> str:
+    """Which transforms fdd_cpu_asis runs: "fftw3f" (the host's libfftw3f.so.3,
+    as the reference links) or "own" (fft_cpu.c)."""
+    return lib().oracle_fdd_cpu_fft_backend().decode()
 
 
 def fdd_cpu_asis_fp64(nchans, dt, delays, dms, nsamps, max_delay_, data_u8) -> np.ndarray:

@@ -18,11 +18,16 @@
  *   - the transposed spectra live on the heap (the reference declares two
  *     nfreq x nchan float VLAs on the stack, :289-290, which overflow the
  *     default 8 MiB stack beyond toy sizes);
- *   - FFTW is replaced by oracle_rfft_f / oracle_irfft_f (fft_cpu.c), so the
- *     float rounding of the transforms differs from FFTW's.
+ *   - FFTW (src/fdd/FDDCPUPlan.cpp:949-1015: one FFTW_ESTIMATE plan per
+ *     call, rows executed in parallel) is loaded at run time with
+ *     dlopen("libfftw3f.so.3") when the host has it; otherwise
+ *     oracle_rfft_f / oracle_irfft_f (fft_cpu.c) stand in, whose float
+ *     rounding differs from FFTW's. oracle_fdd_cpu_fft_backend() names the
+ *     one used.
  * A DM subset [dm_first, dm_first+dm_count) may be computed (the DM loop is
  * independent per DM, :305-402); the timing sample uses this.
  */
+#include <dlfcn.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -34,6 +39,39 @@ void oracle_rfft_f(unsigned long n, const float* in, float* out);
 void oracle_irfft_f(unsigned long n, const float* in, float* out);
 
 static unsigned long round_up_ul(unsigned long a, unsigned long b) { return ((a + b - 1) / b) * b; }
+
+/* FFTW single-precision entry points (the public fftw3 API), if present. */
+typedef void* fftwf_plan_t;
+static struct {
+    int tried, ok;
+    fftwf_plan_t (*plan_r2c)(int n, float* in, float* out, unsigned flags);
+    fftwf_plan_t (*plan_c2r)(int n, float* in, float* out, unsigned flags);
+    void (*exec_r2c)(const fftwf_plan_t p, float* in, float* out);
+    void (*exec_c2r)(const fftwf_plan_t p, float* in, float* out);
+    void (*destroy)(fftwf_plan_t p);
+} g_fftw;
+#define ORACLE_FFTW_ESTIMATE (1U << 6)
+
+static void load_fftw(void)
+{
+    if (g_fftw.tried) return;
+    g_fftw.tried = 1;
+    void* h = dlopen("libfftw3f.so.3", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return;
+    *(void**)&g_fftw.plan_r2c = dlsym(h, "fftwf_plan_dft_r2c_1d");
+    *(void**)&g_fftw.plan_c2r = dlsym(h, "fftwf_plan_dft_c2r_1d");
+    *(void**)&g_fftw.exec_r2c = dlsym(h, "fftwf_execute_dft_r2c");
+    *(void**)&g_fftw.exec_c2r = dlsym(h, "fftwf_execute_dft_c2r");
+    *(void**)&g_fftw.destroy = dlsym(h, "fftwf_destroy_plan");
+    g_fftw.ok = g_fftw.plan_r2c && g_fftw.plan_c2r && g_fftw.exec_r2c && g_fftw.exec_c2r && g_fftw.destroy;
+}
+
+/* "fftw3f" when the transforms run through the host's FFTW, else "own". */
+const char* oracle_fdd_cpu_fft_backend(void)
+{
+    load_fftw();
+    return g_fftw.ok ? "fftw3f" : "own";
+}
 
 static double now_s(void)
 {
@@ -76,13 +114,24 @@ int oracle_fdd_cpu_asis(unsigned long nchan, float dt, const float* delays,
                 ((float)in[t * nchan + (unsigned long)c] - offset) / scale;
 
     /* fft_r2c_inplace(nsamp_fft, nchan, nsamp_padded) */
+    load_fftw();
+    if (g_fftw.ok) {
+        fftwf_plan_t plan = g_fftw.plan_r2c((int)nsamp_fft, data_nu, data_nu, ORACLE_FFTW_ESTIMATE);
 #pragma omp parallel for
-    for (c = 0; c < (long)nchan; ++c) {
-        float* row = data_nu + (unsigned long)c * nsamp_padded;
-        float* tmp = (float*)malloc((nsamp_fft + 2) * sizeof(float));
-        oracle_rfft_f(nsamp_fft, row, tmp);
-        memcpy(row, tmp, (nsamp_fft + 2) * sizeof(float));
-        free(tmp);
+        for (c = 0; c < (long)nchan; ++c) {
+            float* row = data_nu + (unsigned long)c * nsamp_padded;
+            g_fftw.exec_r2c(plan, row, row);
+        }
+        g_fftw.destroy(plan);
+    } else {
+#pragma omp parallel for
+        for (c = 0; c < (long)nchan; ++c) {
+            float* row = data_nu + (unsigned long)c * nsamp_padded;
+            float* tmp = (float*)malloc((nsamp_fft + 2) * sizeof(float));
+            oracle_rfft_f(nsamp_fft, row, tmp);
+            memcpy(row, tmp, (nsamp_fft + 2) * sizeof(float));
+            free(tmp);
+        }
     }
 
     double t_fft = now_s();
@@ -139,16 +188,24 @@ int oracle_fdd_cpu_asis(unsigned long nchan, float dt, const float* delays,
 
     /* fft_c2r_inplace(nsamp_fft, ndm, nsamp_padded) with the 1/n scale, then
      * copy_data of the first nsamp_computed samples per DM */
+    fftwf_plan_t iplan = g_fftw.ok ? g_fftw.plan_c2r((int)nsamp_fft, data_dm, data_dm, ORACLE_FFTW_ESTIMATE) : 0;
 #pragma omp parallel for
     for (i = 0; i < (long)dm_count; ++i) {
         float* row = data_dm + (unsigned long)i * nsamp_padded;
-        float* tmp = (float*)malloc(nsamp_fft * sizeof(float));
-        oracle_irfft_f(nsamp_fft, row, tmp);
-        double scale_n = 1.0 / (double)nsamp_fft;
-        for (unsigned long t = 0; t < nsamp_computed; ++t)
-            out[(unsigned long)i * nsamp_computed + t] = (float)((double)tmp[t] * scale_n);
-        free(tmp);
+        const double scale_n = 1.0 / (double)nsamp_fft;
+        if (iplan) {
+            g_fftw.exec_c2r(iplan, row, row);
+            for (unsigned long t = 0; t < nsamp_computed; ++t)
+                out[(unsigned long)i * nsamp_computed + t] = (float)((double)row[t] * scale_n);
+        } else {
+            float* tmp = (float*)malloc(nsamp_fft * sizeof(float));
+            oracle_irfft_f(nsamp_fft, row, tmp);
+            for (unsigned long t = 0; t < nsamp_computed; ++t)
+                out[(unsigned long)i * nsamp_computed + t] = (float)((double)tmp[t] * scale_n);
+            free(tmp);
+        }
     }
+    if (iplan) g_fftw.destroy(iplan);
     free(data_dm);
     if (times) {
         times[0] = t_fft - t_start;
