@@ -139,10 +139,13 @@ hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint
 // delay32 (the delay table, nchan rounded up to 32 entries, zero padded) the
 // integer delays are computed in the kernel as (int)(dm * delay[c]) and q is
 // not read.
+// cfg selects the workgroup shape: 0 = 8 waves of 2 16-row DM tiles and 4
+// bin slots; 1 = 4 waves of 4 DM tiles and 2 bin slots (half the B-operand
+// LDS reads per product, two workgroups per CU).
 hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2* X, size_t Xp, const uint32_t* q,
                                   size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
                                   float xscale, float2* Y, size_t Yp, const float2* ph_packed,
-                                  const float* dmlist, const float* delay32, hipStream_t stream);
+                                  const float* dmlist, const float* delay32, int cfg, hipStream_t stream);
 // N-tiles (8 chunks each) per workgroup dd_fdd_contract_mfma16 uses for
 // nchunk chunks, and the number of chunk groups.
 int dd_fdd_mfma16_tiles(size_t nchunk, int prec, int* ntl, size_t* ngroups);

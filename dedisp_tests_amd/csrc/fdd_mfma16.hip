@@ -48,12 +48,7 @@ typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 
-constexpr int WAVES = 8;        // waves per workgroup
-constexpr int MT = 2;           // 16-row DM tiles per wave
-constexpr int NT = WAVES * 64;  // threads
-constexpr int DPB = WAVES * MT * 16;  // DMs per workgroup
 constexpr int KS = 32;          // channels per K-step
-constexpr int NS = 4;           // bin slots per workgroup
 
 __device__ __forceinline__ uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
 
@@ -65,6 +60,33 @@ __device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uin
     const float ra = a - __builtin_bit_cast(float, fbits(a) & 0xffff0000u);
     const float rb = b - __builtin_bit_cast(float, fbits(b) & 0xffff0000u);
     const bf16x2 l = {(__bf16)ra, (__bf16)rb};
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// hi: the round-to-nearest bfloat16 of a (low half) and b (high half); lo:
+// the rounded bfloat16 of the remainders, a - hi(a) from one bfloat16 dot
+// product (-1 * hi(a) + 0 * hi(b) + a, exact: the difference is representable).
+// Four instructions per pair instead of six. The selector pairs (-1, 0) and
+// (0, -1) come in registers (DotSel): as an inline constant the compiler's
+// encoding of -1.0 for a bfloat16 operand computes a different product on
+// gfx950 (measured), a register operand the exact one.
+struct DotSel {
+    bf16x2 m1, m2;
+    __device__ __forceinline__ DotSel()
+    {
+        uint32_t r1 = 0x0000bf80u, r2 = 0xbf800000u;  // (-1, 0), (0, -1)
+        asm volatile("" : "+v"(r1), "+v"(r2));
+        m1 = __builtin_bit_cast(bf16x2, r1);
+        m2 = __builtin_bit_cast(bf16x2, r2);
+    }
+};
+__device__ __forceinline__ void split_bf16x2_rn(float a, float b, uint32_t& hi, uint32_t& lo, const DotSel& ds)
+{
+    const bf16x2 h = {(__bf16)a, (__bf16)b};
+    const float ra = __builtin_amdgcn_fdot2_f32_bf16(h, ds.m1, a, false);
+    const float rb = __builtin_amdgcn_fdot2_f32_bf16(h, ds.m2, b, false);
+    const bf16x2 l = {(__bf16)ra, (__bf16)rb};
+    hi = __builtin_bit_cast(uint32_t, h);
     lo = __builtin_bit_cast(uint32_t, l);
 }
 
@@ -99,26 +121,49 @@ struct Planes<2> {  // float16: [Xr|Xi], [-Xi|Xr]
     using V = f16x8;
 };
 
-template <int PREC, int NTL>
+// Staging layout. A staging item is (slot s, channel pair cp, chunk jj),
+// s fastest, then cp: a wave's 64 lanes write 16 channel pairs of NS slots
+// (of one or two chunks). The slot stride is an odd multiple of 128/NS bytes
+// beyond a multiple of 128 (NCOL*COLB is one), so the 32 lanes of a half-wave
+// write 32 distinct banks.
+template <int PREC, int NTL, int NS, int NT>
 struct Geo {
     static constexpr int COLB = KS * Planes<PREC>::ELEM + 16;  // bytes per staged column
     static constexpr int CG = 8 * NTL;                    // chunks per workgroup
     static constexpr int NCOL = 16 * NTL;                 // columns per bin slot
-    static constexpr int SSTR = NCOL * COLB + 16;         // bytes per slot (+16: staging writes spread over banks)
+    static constexpr int SSTR = NCOL * COLB + 128 / NS;   // bytes per slot
     static constexpr int PSTR = NS * SSTR;                // bytes per plane
     static constexpr int ITEMS = NS * CG * (KS / 2);      // staging items per K-step (channel pairs)
     static constexpr int IPT = ITEMS / NT;                // items per thread
     static_assert(IPT * NT == ITEMS, "staging items divide over the threads");
+    static_assert((NCOL * COLB) % 128 == 0 && (NS == 2 || NS == 4), "bank spread of the staging writes");
 };
 
-template <int PREC, int NTL, bool MIRROR>
-__global__ __launch_bounds__(NT, 1) void fdd_mfma16_k(
+// WAVES waves per workgroup, each MT 16-row DM tiles (sharing every B read);
+// NS bin slots (with MIRROR NS/2 direct bins and their mirrors).
+//
+// PIPE (bfloat16 and float16 with the delays computed in the kernel, 8 waves
+// of 2 DM tiles): software pipelined. A K-step is NDIR units, one computed bin
+// each (its direct slot and, with MIRROR, its mirror slot); between the
+// product groups of a unit the wave generates pieces of the next unit's
+// phasors (the next K-step's first bin in the last unit) and stages half of
+// the next K-step's spectra after the first and the last unit, so that the
+// phasor arithmetic issues beside the matrix products instead of before them.
+// Measured on 1024 ch x 2048 DMs x 23 chunks of 65536 (bfloat16 with mirror
+// bins): 45.6 ms against 49.6 ms for cfg 0; 4 waves of 4 tiles (one wave per
+// SIMD, 512 registers) ran 56.5 ms, its waits on the LDS exposed.
+template <int PREC, int NTL, bool MIRROR, int WAVES, int MT, int NS, bool PIPE = false>
+__global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
     const float2* __restrict__ X, size_t Xp, const uint32_t* __restrict__ q, size_t qpitch,
     uint32_t ndm, uint32_t nchan, uint32_t nchunk, uint32_t nfft, float xscale, float yscale,
     float2* __restrict__ Y, size_t Yp, uint32_t nbb, uint32_t ndb, const float2* __restrict__ zph,
     const float* __restrict__ dmlist, const float* __restrict__ dtab)
 {
-    using G = Geo<PREC, NTL>;
+    static_assert(!PIPE || (PREC != 0 && NS == 4 && WAVES == 8), "pipelined form: bfloat16 / float16");
+    constexpr int NT = WAVES * 64;        // threads
+    constexpr int DPB = WAVES * MT * 16;  // DMs per workgroup
+    constexpr int NDIR = MIRROR ? NS / 2 : NS;  // bins whose phasors are computed
+    using G = Geo<PREC, NTL, NS, NT>;
     using PL = Planes<PREC>;
     using V = typename PL::V;
     constexpr int COLB = G::COLB;
@@ -143,28 +188,29 @@ __global__ __launch_bounds__(NT, 1) void fdd_mfma16_k(
     const uint32_t cgrp = rest / nbb;
     const uint32_t ch0 = cgrp * G::CG;
     // bin of slot s and whether it is computed and written here
-    const uint32_t k0 = MIRROR ? 2 * bblk : NS * bblk;
+    const uint32_t k0 = NDIR * bblk;
     auto slot_bin = [&](int s) -> uint32_t {
-        if (MIRROR) return s < 2 ? k0 + s : M - (k0 + s - 2);
+        if (MIRROR) return s < NDIR ? k0 + s : M - (k0 + s - NDIR);
         return k0 + s;
     };
     auto slot_valid = [&](int s) -> bool {
-        if (MIRROR) return s < 2 ? (k0 + s <= M / 2) : (k0 + s - 2 < M / 2);
+        if (MIRROR) return s < NDIR ? (k0 + s <= M / 2) : (k0 + s - NDIR < M / 2);
         return k0 + s <= M;
     };
 
-    // ---- staging role: items (slot s, chunk jj, channel pair cp) ----
+    // ---- staging role: items (slot s, channel pair cp, chunk jj) ----
     // Every load is unconditional and in bounds (X has its channel count
     // padded to a multiple of 32 with zero rows; out-of-range bins and chunks
     // read a valid element and are multiplied by 0).
     const size_t rs = (size_t)nchunk * Xp;  // channel stride of X
+    const DotSel dsel;  // used by the pipelined bfloat16 form
     const float2* src[G::IPT];
     float sscale[G::IPT];
     int soff[G::IPT];
 #pragma unroll
     for (int u = 0; u < G::IPT; ++u) {
         const int it = tid + u * NT;
-        const int s = it % NS, jj = (it / NS) % G::CG, cp = it / (NS * G::CG);
+        const int s = it % NS, cp = (it / NS) % (KS / 2), jj = it / (NS * (KS / 2));
         const uint32_t chunk = ch0 + jj;
         const uint32_t bin = min(slot_bin(s), M);
         src[u] = X + ((size_t)(2 * cp) * nchunk + min(chunk, nchunk - 1)) * Xp + bin;
@@ -180,37 +226,44 @@ __global__ __launch_bounds__(NT, 1) void fdd_mfma16_k(
             xv[u][1] = src[u][kofs + rs];
         }
     };
-    auto store_x = [&](char* buf) {
-#pragma unroll
-        for (int u = 0; u < G::IPT; ++u) {
-            const float xr0 = xv[u][0].x * sscale[u], xi0 = xv[u][0].y * sscale[u];
-            const float xr1 = xv[u][1].x * sscale[u], xi1 = xv[u][1].y * sscale[u];
-            char* b = buf + soff[u];
-            if constexpr (PREC == 0) {
-                *reinterpret_cast<float2*>(b) = make_float2(xr0, xr1);
-                *reinterpret_cast<float2*>(b + 8 * COLB) = make_float2(xi0, xi1);
-                *reinterpret_cast<float2*>(b + G::PSTR) = make_float2(-xi0, -xi1);
-                *reinterpret_cast<float2*>(b + G::PSTR + 8 * COLB) = make_float2(xr0, xr1);
-            } else if constexpr (PREC == 1) {
-                uint32_t rh, rl, ih, il;
+    // staged form of item u (channels 2cp, 2cp+1 of one chunk and slot)
+    auto store_item = [&](char* buf, int u, float2 x0, float2 x1) __attribute__((always_inline)) {
+        const float xr0 = x0.x * sscale[u], xi0 = x0.y * sscale[u];
+        const float xr1 = x1.x * sscale[u], xi1 = x1.y * sscale[u];
+        char* b = buf + soff[u];
+        if constexpr (PREC == 0) {
+            *reinterpret_cast<float2*>(b) = make_float2(xr0, xr1);
+            *reinterpret_cast<float2*>(b + 8 * COLB) = make_float2(xi0, xi1);
+            *reinterpret_cast<float2*>(b + G::PSTR) = make_float2(-xi0, -xi1);
+            *reinterpret_cast<float2*>(b + G::PSTR + 8 * COLB) = make_float2(xr0, xr1);
+        } else if constexpr (PREC == 1) {
+            uint32_t rh, rl, ih, il;
+            if constexpr (PIPE) {
+                split_bf16x2_rn(xr0, xr1, rh, rl, dsel);
+                split_bf16x2_rn(xi0, xi1, ih, il, dsel);
+            } else {
                 split_bf16x2(xr0, xr1, rh, rl);
                 split_bf16x2(xi0, xi1, ih, il);
-                *reinterpret_cast<uint32_t*>(b) = rh;
-                *reinterpret_cast<uint32_t*>(b + 8 * COLB) = ih;
-                *reinterpret_cast<uint32_t*>(b + G::PSTR) = rl;
-                *reinterpret_cast<uint32_t*>(b + G::PSTR + 8 * COLB) = il;
-                *reinterpret_cast<uint32_t*>(b + 2 * G::PSTR) = ih ^ 0x80008000u;
-                *reinterpret_cast<uint32_t*>(b + 2 * G::PSTR + 8 * COLB) = rh;
-                *reinterpret_cast<uint32_t*>(b + 3 * G::PSTR) = il ^ 0x80008000u;
-                *reinterpret_cast<uint32_t*>(b + 3 * G::PSTR + 8 * COLB) = rl;
-            } else {
-                const uint32_t r = pack_f16x2(xr0, xr1), i = pack_f16x2(xi0, xi1);
-                *reinterpret_cast<uint32_t*>(b) = r;
-                *reinterpret_cast<uint32_t*>(b + 8 * COLB) = i;
-                *reinterpret_cast<uint32_t*>(b + G::PSTR) = i ^ 0x80008000u;
-                *reinterpret_cast<uint32_t*>(b + G::PSTR + 8 * COLB) = r;
             }
+            *reinterpret_cast<uint32_t*>(b) = rh;
+            *reinterpret_cast<uint32_t*>(b + 8 * COLB) = ih;
+            *reinterpret_cast<uint32_t*>(b + G::PSTR) = rl;
+            *reinterpret_cast<uint32_t*>(b + G::PSTR + 8 * COLB) = il;
+            *reinterpret_cast<uint32_t*>(b + 2 * G::PSTR) = ih ^ 0x80008000u;
+            *reinterpret_cast<uint32_t*>(b + 2 * G::PSTR + 8 * COLB) = rh;
+            *reinterpret_cast<uint32_t*>(b + 3 * G::PSTR) = il ^ 0x80008000u;
+            *reinterpret_cast<uint32_t*>(b + 3 * G::PSTR + 8 * COLB) = rl;
+        } else {
+            const uint32_t r = pack_f16x2(xr0, xr1), i = pack_f16x2(xi0, xi1);
+            *reinterpret_cast<uint32_t*>(b) = r;
+            *reinterpret_cast<uint32_t*>(b + 8 * COLB) = i;
+            *reinterpret_cast<uint32_t*>(b + G::PSTR) = i ^ 0x80008000u;
+            *reinterpret_cast<uint32_t*>(b + G::PSTR + 8 * COLB) = r;
         }
+    };
+    auto store_x = [&](char* buf) {
+#pragma unroll
+        for (int u = 0; u < G::IPT; ++u) store_item(buf, u, xv[u][0], xv[u][1]);
     };
 
     // ---- compute role: DM rows lane%16 of tiles mt, channels 8g .. 8g+7 ----
@@ -272,176 +325,359 @@ __global__ __launch_bounds__(NT, 1) void fdd_mfma16_k(
             }
         }
     };
-    load_x(0);
-    load_q(0);
-    store_x(xs);
-    __syncthreads();
+    if constexpr (!PIPE) {
+        load_x(0);
+        load_q(0);
+        store_x(xs);
+        __syncthreads();
 
-    for (uint32_t ks = 0; ks < nks; ++ks) {
-        const char* cur = xs + (ks & 1) * BUF;
-        const bool more = ks + 1 < nks;
-        u32x4 qc[MT][2];
-        delays_now(qc);
-        // prefetch K-step min(ks + 1, nks - 1): the last iteration re-reads
-        // its own K-step (in bounds, unused)
-        const uint32_t kn = more ? ks + 1 : ks;
-        load_x((size_t)kn * kstep);
-        load_q(kn);
+        for (uint32_t ks = 0; ks < nks; ++ks) {
+            const char* cur = xs + (ks & 1) * BUF;
+            const bool more = ks + 1 < nks;
+            u32x4 qc[MT][2];
+            delays_now(qc);
+            // prefetch K-step min(ks + 1, nks - 1): the last iteration re-reads
+            // its own K-step (in bounds, unused)
+            const uint32_t kn = more ? ks + 1 : ks;
+            load_x((size_t)kn * kstep);
+            load_q(kn);
 
-        // sign masks of the mirror phasors: (-1)^d per channel, d odd <=> q odd
-        uint32_t smask[MT][4];
-        if constexpr (MIRROR) {
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t qa = qc[mt][j >> 1][2 * (j & 1)], qb = qc[mt][j >> 1][2 * (j & 1) + 1];
-                    smask[mt][j] = ((qa & 1u) << 15) | ((qb & 1u) << 31);
-                }
-        }
-
-        constexpr int NDIR = MIRROR ? 2 : NS;
-        if constexpr (PREC == 0) {
-            // float32: A is one phasor component per lane and 16 x 16 x 4
-            // MFMA; MFMA e of a K-step sums channels 8g + e of the lane groups
-#pragma unroll
-            for (int b = 0; b < NDIR; ++b) {
-                const uint32_t kb = k0 + b;
-                float pr[MT][8], pim[MT][8];
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const uint32_t qq = qc[mt][e >> 2][e & 3];
-                        const float a = (float)(__umul24(kb, qq) & mask) * inv;
-                        pr[mt][e] = __builtin_amdgcn_cosf(a);
-                        pim[mt][e] = __builtin_amdgcn_sinf(a);
-                    }
-#pragma unroll
-                for (int pass = 0; pass < (MIRROR ? 2 : 1); ++pass) {
-                    const int s = pass ? 2 + b : b;
-                    if (pass) {
-                        // re' = (-1)^d re, im' = -(-1)^d im
-#pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                            for (int e = 0; e < 8; ++e) {
-                                const uint32_t sg = (qc[mt][e >> 2][e & 3] & 1u) << 31;
-                                pr[mt][e] = __builtin_bit_cast(float, fbits(pr[mt][e]) ^ sg);
-                                pim[mt][e] = __builtin_bit_cast(float, fbits(pim[mt][e]) ^ sg ^ 0x80000000u);
-                            }
-                    }
-                    const char* sb = cur + s * G::SSTR + b_off;
-#pragma unroll
-                    for (int t = 0; t < NTL; ++t) {
-                        float xr[8], xi[8];
-                        *reinterpret_cast<float4*>(xr) = *reinterpret_cast<const float4*>(sb + t * 16 * COLB);
-                        *reinterpret_cast<float4*>(xr + 4) =
-                            *reinterpret_cast<const float4*>(sb + t * 16 * COLB + 16);
-                        *reinterpret_cast<float4*>(xi) =
-                            *reinterpret_cast<const float4*>(sb + G::PSTR + t * 16 * COLB);
-                        *reinterpret_cast<float4*>(xi + 4) =
-                            *reinterpret_cast<const float4*>(sb + G::PSTR + t * 16 * COLB + 16);
-#pragma unroll
-                        for (int e = 0; e < 8; ++e)
-#pragma unroll
-                            for (int mt = 0; mt < MT; ++mt) {
-                                acc[s][mt][t] =
-                                    __builtin_amdgcn_mfma_f32_16x16x4f32(pr[mt][e], xr[e], acc[s][mt][t], 0, 0, 0);
-                                acc[s][mt][t] =
-                                    __builtin_amdgcn_mfma_f32_16x16x4f32(pim[mt][e], xi[e], acc[s][mt][t], 0, 0, 0);
-                            }
-                    }
-                }
-            }
-        } else {
-    #pragma unroll
-            for (int b = 0; b < NDIR; ++b) {
-                const uint32_t kb = k0 + b;
-                // phasor planes of bin kb for the two DM tiles
-                uint32_t A[MT][PL::N][4];
+            // sign masks of the mirror phasors: (-1)^d per channel, d odd <=> q odd
+            uint32_t smask[MT][4];
+            if constexpr (MIRROR) {
     #pragma unroll
                 for (int mt = 0; mt < MT; ++mt)
     #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        float cr[2], ci[2];
-    #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            const uint32_t qq = qc[mt][j >> 1][2 * (j & 1) + h];
-                            const float a = (float)(__umul24(kb, qq) & mask) * inv;
-                            cr[h] = __builtin_amdgcn_cosf(a);
-                            ci[h] = __builtin_amdgcn_sinf(a);
-                        }
-                        if constexpr (PREC == 1) {
-                            split_bf16x2(cr[0], cr[1], A[mt][0][j], A[mt][1][j]);
-                            split_bf16x2(ci[0], ci[1], A[mt][2][j], A[mt][3][j]);
-                        } else {
-                            A[mt][0][j] = pack_f16x2(cr[0], cr[1]);
-                            A[mt][1][j] = pack_f16x2(ci[0], ci[1]);
-                        }
+                        const uint32_t qa = qc[mt][j >> 1][2 * (j & 1)], qb = qc[mt][j >> 1][2 * (j & 1) + 1];
+                        smask[mt][j] = ((qa & 1u) << 15) | ((qb & 1u) << 31);
                     }
-                // the direct slot, then (MIRROR) its mirror with the signs flipped
+            }
+
+            if constexpr (PREC == 0) {
+                // float32: A is one phasor component per lane and 16 x 16 x 4
+                // MFMA; MFMA e of a K-step sums channels 8g + e of the lane groups
     #pragma unroll
-                for (int pass = 0; pass < (MIRROR ? 2 : 1); ++pass) {
-                    const int s = pass ? 2 + b : b;
-                    if (pass) {
-    #pragma unroll
-                        for (int mt = 0; mt < MT; ++mt)
-    #pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                // re' = (-1)^d re, im' = -(-1)^d im
-    #pragma unroll
-                                for (int pl = 0; pl < PL::N; ++pl)
-                                    A[mt][pl][j] ^= smask[mt][j] ^ ((pl >= PL::N / 2) ? 0x80008000u : 0u);
-                            }
-                    }
-                    V a[MT][PL::N];
+                for (int b = 0; b < NDIR; ++b) {
+                    const uint32_t kb = k0 + b;
+                    float pr[MT][8], pim[MT][8];
     #pragma unroll
                     for (int mt = 0; mt < MT; ++mt)
     #pragma unroll
-                        for (int pl = 0; pl < PL::N; ++pl) a[mt][pl] = __builtin_bit_cast(V, A[mt][pl]);
-                    const char* sb = cur + s * G::SSTR + b_off;
+                        for (int e = 0; e < 8; ++e) {
+                            const uint32_t qq = qc[mt][e >> 2][e & 3];
+                            const float a = (float)(__umul24(kb, qq) & mask) * inv;
+                            pr[mt][e] = __builtin_amdgcn_cosf(a);
+                            pim[mt][e] = __builtin_amdgcn_sinf(a);
+                        }
     #pragma unroll
-                    for (int t = 0; t < NTL; ++t) {
-                        V x[PL::N];
-    #pragma unroll
-                        for (int pl = 0; pl < PL::N; ++pl)
-                            x[pl] = *reinterpret_cast<const V*>(sb + pl * G::PSTR + t * 16 * COLB);
-                        if constexpr (PREC == 1) {
-                            // planes: 0 Ar hi, 1 Ar lo, 2 Ai hi, 3 Ai lo; x: 0 BR hi, 1 BR lo, 2 BI hi, 3 BI lo
-    #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
+                    for (int pass = 0; pass < (MIRROR ? 2 : 1); ++pass) {
+                        const int s = pass ? NDIR + b : b;
+                        if (pass) {
+                            // re' = (-1)^d re, im' = -(-1)^d im
     #pragma unroll
                             for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[2], acc[s][mt][t], 0, 0, 0);
     #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[1], acc[s][mt][t], 0, 0, 0);
+                                for (int e = 0; e < 8; ++e) {
+                                    const uint32_t sg = (qc[mt][e >> 2][e & 3] & 1u) << 31;
+                                    pr[mt][e] = __builtin_bit_cast(float, fbits(pr[mt][e]) ^ sg);
+                                    pim[mt][e] = __builtin_bit_cast(float, fbits(pim[mt][e]) ^ sg ^ 0x80000000u);
+                                }
+                        }
+                        const char* sb = cur + s * G::SSTR + b_off;
     #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[3], acc[s][mt][t], 0, 0, 0);
+                        for (int t = 0; t < NTL; ++t) {
+                            float xr[8], xi[8];
+                            *reinterpret_cast<float4*>(xr) = *reinterpret_cast<const float4*>(sb + t * 16 * COLB);
+                            *reinterpret_cast<float4*>(xr + 4) =
+                                *reinterpret_cast<const float4*>(sb + t * 16 * COLB + 16);
+                            *reinterpret_cast<float4*>(xi) =
+                                *reinterpret_cast<const float4*>(sb + G::PSTR + t * 16 * COLB);
+                            *reinterpret_cast<float4*>(xi + 4) =
+                                *reinterpret_cast<const float4*>(sb + G::PSTR + t * 16 * COLB + 16);
     #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][1], x[0], acc[s][mt][t], 0, 0, 0);
+                            for (int e = 0; e < 8; ++e)
     #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt) {
+                                    acc[s][mt][t] =
+                                        __builtin_amdgcn_mfma_f32_16x16x4f32(pr[mt][e], xr[e], acc[s][mt][t], 0, 0, 0);
+                                    acc[s][mt][t] =
+                                        __builtin_amdgcn_mfma_f32_16x16x4f32(pim[mt][e], xi[e], acc[s][mt][t], 0, 0, 0);
+                                }
+                        }
+                    }
+                }
+            } else {
+        #pragma unroll
+                for (int b = 0; b < NDIR; ++b) {
+                    const uint32_t kb = k0 + b;
+                    // phasor planes of bin kb for the two DM tiles
+                    uint32_t A[MT][PL::N][4];
+        #pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+        #pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            float cr[2], ci[2];
+        #pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const uint32_t qq = qc[mt][j >> 1][2 * (j & 1) + h];
+                                const float a = (float)(__umul24(kb, qq) & mask) * inv;
+                                cr[h] = __builtin_amdgcn_cosf(a);
+                                ci[h] = __builtin_amdgcn_sinf(a);
+                            }
+                            if constexpr (PREC == 1) {
+                                split_bf16x2(cr[0], cr[1], A[mt][0][j], A[mt][1][j]);
+                                split_bf16x2(ci[0], ci[1], A[mt][2][j], A[mt][3][j]);
+                            } else {
+                                A[mt][0][j] = pack_f16x2(cr[0], cr[1]);
+                                A[mt][1][j] = pack_f16x2(ci[0], ci[1]);
+                            }
+                        }
+                    // the direct slot, then (MIRROR) its mirror with the signs flipped
+        #pragma unroll
+                    for (int pass = 0; pass < (MIRROR ? 2 : 1); ++pass) {
+                        const int s = pass ? NDIR + b : b;
+                        if (pass) {
+        #pragma unroll
                             for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][3], x[2], acc[s][mt][t], 0, 0, 0);
-                        } else {
-    #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
-    #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][1], x[1], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    // re' = (-1)^d re, im' = -(-1)^d im
+        #pragma unroll
+                                    for (int pl = 0; pl < PL::N; ++pl)
+                                        A[mt][pl][j] ^= smask[mt][j] ^ ((pl >= PL::N / 2) ? 0x80008000u : 0u);
+                                }
+                        }
+                        V a[MT][PL::N];
+        #pragma unroll
+                        for (int mt = 0; mt < MT; ++mt)
+        #pragma unroll
+                            for (int pl = 0; pl < PL::N; ++pl) a[mt][pl] = __builtin_bit_cast(V, A[mt][pl]);
+                        const char* sb = cur + s * G::SSTR + b_off;
+        #pragma unroll
+                        for (int t = 0; t < NTL; ++t) {
+                            V x[PL::N];
+        #pragma unroll
+                            for (int pl = 0; pl < PL::N; ++pl)
+                                x[pl] = *reinterpret_cast<const V*>(sb + pl * G::PSTR + t * 16 * COLB);
+                            if constexpr (PREC == 1) {
+                                // planes: 0 Ar hi, 1 Ar lo, 2 Ai hi, 3 Ai lo; x: 0 BR hi, 1 BR lo, 2 BI hi, 3 BI lo
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[2], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[1], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[3], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][1], x[0], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][3], x[2], acc[s][mt][t], 0, 0, 0);
+                            } else {
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
+        #pragma unroll
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][1], x[1], acc[s][mt][t], 0, 0, 0);
+                            }
                         }
                     }
                 }
             }
-        }
 
-        if (more) store_x(xs + ((ks + 1) & 1) * BUF);
+            if (more) store_x(xs + ((ks + 1) & 1) * BUF);
+            __syncthreads();
+        }
+    } else {
+        // Delays of the lane's 8 channels for its MT tiles: (int)(dm * delay[c])
+        // in float (the conversion truncates), the reference's integer delay.
+        constexpr int NU = NDIR;                // units (computed bins) per K-step
+        uint32_t d[MT][8];
+        float4 dlv[2];
+        auto load_dl = [&](uint32_t ks) __attribute__((always_inline)) {
+            const float* dp = dtab + ks * KS + 8 * g;
+            dlv[0] = *reinterpret_cast<const float4*>(dp);
+            dlv[1] = *reinterpret_cast<const float4*>(dp + 4);
+        };
+        auto make_d = [&]() __attribute__((always_inline)) {
+            const float dl[8] = {dlv[0].x, dlv[0].y, dlv[0].z, dlv[0].w, dlv[1].x, dlv[1].y, dlv[1].z, dlv[1].w};
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) d[mt][e] = (uint32_t)(int)(dmv[mt] * dl[e]);
+        };
+        // Phase of bin kb as a float in [1, 2): (kb << (23 - lgn)) * d keeps
+        // (kb*d mod nfft) / nfft in mantissa bits 22 .. 23-lgn (exact), and
+        // v_cos_f32 / v_sin_f32 take revolutions, so 1 + phase is as good as
+        // the phase itself: one multiply and one and-or per phasor.
+        const uint32_t sh = 23u - (31u - (uint32_t)__builtin_clz(nfft));
+        using APl = uint32_t[MT][PL::N][4];
+        // piece pc = 4*mt + j of the phasor planes of bin kb: two channels of tile mt
+        auto gen_piece = [&](uint32_t kb, APl& A, int pc) __attribute__((always_inline)) {
+            const int mt = pc >> 2, j = pc & 3;
+            const uint32_t kbs = kb << sh;
+            float cr[2], ci[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t n = __umul24(kbs, d[mt][2 * j + h]);
+                const float x = __builtin_bit_cast(float, (n & 0x7fffffu) | 0x3f800000u);
+                cr[h] = __builtin_amdgcn_cosf(x);
+                ci[h] = __builtin_amdgcn_sinf(x);
+            }
+            if constexpr (PREC == 1) {
+                split_bf16x2_rn(cr[0], cr[1], A[mt][0][j], A[mt][1][j], dsel);
+                split_bf16x2_rn(ci[0], ci[1], A[mt][2][j], A[mt][3][j], dsel);
+            } else {
+                A[mt][0][j] = pack_f16x2(cr[0], cr[1]);
+                A[mt][1][j] = pack_f16x2(ci[0], ci[1]);
+            }
+        };
+        // the products of slot s, N-tile t with phasor planes A, staged buffer cur
+        auto group_products = [&](const char* cur, int s, int t, const APl& A) __attribute__((always_inline)) {
+            V a[MT][PL::N];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int pl = 0; pl < PL::N; ++pl) a[mt][pl] = __builtin_bit_cast(V, A[mt][pl]);
+            const char* sb = cur + s * G::SSTR + b_off + t * 16 * COLB;
+            V x[PL::N];
+#pragma unroll
+            for (int pl = 0; pl < PL::N; ++pl) x[pl] = *reinterpret_cast<const V*>(sb + pl * G::PSTR);
+            if constexpr (PREC == 1) {
+                // planes: 0 Ar hi, 1 Ar lo, 2 Ai hi, 3 Ai lo; x: 0 BR hi, 1 BR lo, 2 BI hi, 3 BI lo
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[2], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][0], x[1], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][2], x[3], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][1], x[0], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][3], x[2], acc[s][mt][t], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][0], x[0], acc[s][mt][t], 0, 0, 0);
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+                    acc[s][mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[mt][1], x[1], acc[s][mt][t], 0, 0, 0);
+            }
+        };
+        // The spectra are staged in two halves: items [0, IPT/2) loaded at the
+        // top of the K-step and staged after unit 0, the rest loaded then and
+        // staged after the last unit (half the staging registers live at a
+        // time). Element offsets from X + kofs fit 32 bits (the launcher checks).
+        constexpr int IH = (G::IPT + 1) / 2;  // items of the first half
+        uint32_t xoff[G::IPT];
+#pragma unroll
+        for (int u = 0; u < G::IPT; ++u) xoff[u] = (uint32_t)(src[u] - X);
+        const uint32_t rs32 = (uint32_t)rs;
+        float2 xh[IH][2];
+        auto load_half = [&](size_t kofs, int p) __attribute__((always_inline)) {
+            const float2* xb = X + kofs;
+#pragma unroll
+            for (int v = 0; v < IH; ++v)
+                if (p * IH + v < G::IPT) {
+                    xh[v][0] = xb[xoff[p * IH + v]];
+                    xh[v][1] = xb[xoff[p * IH + v] + rs32];
+                }
+        };
+        auto store_half = [&](char* buf, int p) __attribute__((always_inline)) {
+#pragma unroll
+            for (int v = 0; v < IH; ++v)
+                if (p * IH + v < G::IPT) store_item(buf, p * IH + v, xh[v][0], xh[v][1]);
+        };
+        // unit u of K-step ks: the products of bin k0 + u (and its mirror) with
+        // phasors Acur while the phasors of the next unit go into Anext
+        auto unit = [&](uint32_t ks, int u, APl& Acur, APl& Anext) __attribute__((always_inline)) {
+            const char* cur = xs + (ks & 1) * BUF;
+            const bool last = u == NU - 1;
+            // sign masks of the mirror phasors from this K-step's delays:
+            // (-1)^d per channel, d odd <=> q odd
+            uint32_t sm[MT][4];
+            if constexpr (MIRROR) {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) sm[mt][j] = (d[mt][2 * j] & 1u) << 15 | (d[mt][2 * j + 1] << 31);
+            }
+            if (last) {
+                // the next K-step's delays (loaded one K-step ago), then the
+                // load of the one after (the last K-steps re-read the last)
+                make_d();
+                load_dl(min(ks + 2, nks - 1));
+            }
+            // the product groups (slot, N-tile) of the unit in order, the direct
+            // slot's first; the pieces of the next unit's phasors spread
+            // between them, so that they issue beside the matrix products
+            constexpr int SPU = MIRROR ? 2 : 1;
+            constexpr int NG = SPU * NTL, NPC = MT * 4;
+            const uint32_t kbn = last ? k0 : k0 + u + 1;
+#pragma unroll
+            for (int gi = 0; gi < NG; ++gi) {
+#pragma unroll
+                for (int pc = gi * NPC / NG; pc < (gi + 1) * NPC / NG; ++pc) gen_piece(kbn, Anext, pc);
+                if (MIRROR && gi == NTL) {
+                    // the mirror slot: re' = (-1)^d re, im' = -(-1)^d im
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+#pragma unroll
+                            for (int pl = 0; pl < PL::N; ++pl)
+                                Acur[mt][pl][j] ^= sm[mt][j] ^ ((pl >= PL::N / 2) ? 0x80008000u : 0u);
+                }
+                group_products(cur, gi < NTL ? u : NDIR + u, gi % NTL, Acur);
+            }
+            // the next K-step's spectra into the other buffer (the last K-step
+            // stores its own again, unread)
+            char* nbuf = xs + ((ks + 1) & 1) * BUF;
+            if (u == 0) {
+                store_half(nbuf, 0);
+                load_half((size_t)((ks + 1 < nks) ? ks + 1 : ks) * kstep, 1);
+            }
+            if (last) store_half(nbuf, 1);
+        };
+
+        load_dl(0);
+        make_d();
+        load_dl(min(1u, nks - 1));
+        load_half(0, 0);
+        store_half(xs, 0);
+        load_half(0, 1);
+        store_half(xs, 1);
+        APl A0, A1;
+#pragma unroll
+        for (int pc = 0; pc < MT * 4; ++pc) gen_piece(k0, A0, pc);
         __syncthreads();
+        for (uint32_t ks = 0; ks < nks; ++ks) {
+            const uint32_t kn = (ks + 1 < nks) ? ks + 1 : ks;
+            load_half((size_t)kn * kstep, 0);
+            unit(ks, 0, A0, A1);
+            unit(ks, 1, A1, A0);
+            if constexpr (NU == 4) {
+                unit(ks, 2, A0, A1);
+                unit(ks, 3, A1, A0);
+            }
+            __syncthreads();
+        }
     }
 
     // ---- epilogue ----
@@ -494,34 +730,29 @@ __global__ __launch_bounds__(NT, 1) void fdd_mfma16_k(
                             return make_float2(e.x - o.y, e.y + o.x);
                         };
 #pragma unroll
-                        for (int s = 0; s < 2; ++s) {
+                        for (int s = 0; s < NDIR; ++s) {
                             const uint32_t k = k0 + s;
-                            if (slot_valid(s)) dst[k] = zu(k, y[s], k == M / 2 ? y[s] : y[s + 2]);
+                            if (slot_valid(s)) dst[k] = zu(k, y[s], k == M / 2 ? y[s] : y[s + NDIR]);
                         }
 #pragma unroll
-                        for (int s = 2; s < 4; ++s) {
-                            const uint32_t kk = M - (k0 + s - 2);
-                            if (slot_valid(s) && kk < M) dst[kk] = zu(kk, y[s], y[s - 2]);
+                        for (int s = NDIR; s < NS; ++s) {
+                            const uint32_t kk = M - (k0 + s - NDIR);
+                            if (slot_valid(s) && kk < M) dst[kk] = zu(kk, y[s], y[s - NDIR]);
                         }
                     } else {
-                        if (slot_valid(0) && slot_valid(1)) {
-                            *reinterpret_cast<float4*>(dst + k0) = make_float4(
-                                vr[0][rr] * yscale, vi[0][rr] * yscale, vr[1][rr] * yscale, vi[1][rr] * yscale);
-                        } else if (slot_valid(0)) {
-                            dst[k0] = make_float2(vr[0][rr] * yscale, vi[0][rr] * yscale);
-                        }
 #pragma unroll
-                        for (int s = 2; s < 4; ++s)
+                        for (int s = 0; s < NS; ++s)
                             if (slot_valid(s))
                                 dst[slot_bin(s)] = make_float2(vr[s][rr] * yscale, vi[s][rr] * yscale);
                     }
                 } else {
-                    if (k0 + 3 <= M) {
-                        float4* d4 = reinterpret_cast<float4*>(dst + k0);
-                        d4[0] = make_float4(vr[0][rr] * yscale, vi[0][rr] * yscale, vr[1][rr] * yscale,
-                                            vi[1][rr] * yscale);
-                        d4[1] = make_float4(vr[2][rr] * yscale, vi[2][rr] * yscale, vr[3][rr] * yscale,
-                                            vi[3][rr] * yscale);
+                    // consecutive bins k0 .. k0+NS-1: 16-byte stores of bin pairs
+                    if (k0 + NS - 1 <= M) {
+#pragma unroll
+                        for (int s = 0; s < NS; s += 2)
+                            *reinterpret_cast<float4*>(dst + k0 + s) =
+                                make_float4(vr[s][rr] * yscale, vi[s][rr] * yscale, vr[s + 1][rr] * yscale,
+                                            vi[s + 1][rr] * yscale);
                     } else {
 #pragma unroll
                         for (int s = 0; s < NS; ++s)
@@ -555,36 +786,48 @@ extern "C" int dd_fdd_mfma16_tiles(size_t nchunk, int prec, int* ntl_out, size_t
 extern "C" hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2* X, size_t Xp, const uint32_t* q,
                                              size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
                                              float xscale, float2* Y, size_t Yp, const float2* ph_packed,
-                                             const float* dmlist, const float* delay32, hipStream_t stream)
+                                             const float* dmlist, const float* delay32, int cfg,
+                                             hipStream_t stream)
 {
+    if (ph_packed && !mirror) return hipErrorInvalidValue;
     if ((dmlist == nullptr) != (delay32 == nullptr)) return hipErrorInvalidValue;
     if (!dmlist && !q) return hipErrorInvalidValue;
-    if (ph_packed && !mirror) return hipErrorInvalidValue;
     if (ndm == 0 || nchunk == 0) return hipSuccess;
     // layout preconditions of the vector loads and stores
-    if ((!dmlist && (qpitch % 32 || qpitch < (nchan + 31) / 32 * 32)) || Yp % 2 || nfft < 64 || nfft > (size_t(1) << 18) ||
-        (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
+    if ((!dmlist && (qpitch % 32 || qpitch < (nchan + 31) / 32 * 32)) || Yp % 2 || nfft < 64 ||
+        nfft > (size_t(1) << 18) || (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
         return hipErrorInvalidValue;
     if (prec != 0 && prec != 1 && prec != 2) return hipErrorInvalidValue;
+    if (cfg < 0 || cfg > 2 || (cfg == 2 && (prec == 0 || !dmlist))) return hipErrorInvalidValue;
+    // the pipelined form addresses the spectra of a K-step by 32-bit element offsets
+    if (cfg == 2 && (size_t)KS * nchunk * Xp + Xp >= (size_t(1) << 32)) return hipErrorInvalidValue;
     const size_t M = nfft / 2;
-    const uint32_t ndb = (uint32_t)((ndm + DPB - 1) / DPB);
+    // cfg 0: 8 waves x 2 DM tiles, 4 bin slots; cfg 1: 4 waves x 4 DM tiles, 2
+    // bin slots; cfg 2: pipelined, 8 waves x 2 DM tiles, 4 bin slots
+    const size_t waves = (cfg == 1) ? 4 : 8, mt = (cfg == 1) ? 4 : 2, ns = (cfg == 1) ? 2 : 4;
+    const size_t dpb = waves * mt * 16, ndir = mirror ? ns / 2 : ns;
+    const uint32_t ndb = (uint32_t)((ndm + dpb - 1) / dpb);
     int ntl = 1;
     size_t ngroups = 0;
     dd_fdd_mfma16_tiles(nchunk, prec, &ntl, &ngroups);
-    // bin blocks: four consecutive bins, or (mirror) two direct bins k <= M/2
-    const uint32_t nbb = (uint32_t)(mirror ? (M / 2 + 2) / 2 : (M + NS) / NS);
+    // bin blocks: ndir direct bins k <= M/2 (mirror) or ns consecutive bins
+    const uint32_t nbb = (uint32_t)(mirror ? (M / 2 + ndir) / ndir : (M + ns) / ns);
     const size_t nblocks = (size_t)nbb * ndb * ngroups;
     if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;
     const float yscale = 1.0f / xscale;
-#define DD_M16(P, T, MI)                                                                                     \
-    hipLaunchKernelGGL((fdd_mfma16_k<P, T, MI>), dim3((unsigned)nblocks), dim3(NT), 0, stream, X, Xp, q, qpitch, \
-                       (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, (uint32_t)nfft, xscale, yscale, Y, Yp,  \
-                       nbb, ndb, ph_packed, dmlist, delay32)
+#define DD_M16(P, T, MI, W, MT_, NS_, PI)                                                                        \
+    hipLaunchKernelGGL((fdd_mfma16_k<P, T, MI, W, MT_, NS_, PI>), dim3((unsigned)nblocks), dim3(W * 64), 0, stream, \
+                       X, Xp, q, qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, (uint32_t)nfft, xscale, \
+                       yscale, Y, Yp, nbb, ndb, ph_packed, dmlist, delay32)
+#define DD_M16_C(P, T, MI)                                                               \
+    if (cfg == 1) DD_M16(P, T, MI, 4, 4, 2, false);                                      \
+    else if (cfg == 2) DD_M16(P, T, MI, 8, 2, 4, P != 0);                                \
+    else DD_M16(P, T, MI, 8, 2, 4, false);
 #define DD_M16_T(P, MI)                    \
     switch (ntl) {                         \
-    case 1: DD_M16(P, 1, MI); break;       \
-    case 2: DD_M16(P, 2, MI); break;       \
-    default: DD_M16(P, 3, MI); break;      \
+    case 1: DD_M16_C(P, 1, MI) break;      \
+    case 2: DD_M16_C(P, 2, MI) break;      \
+    default: DD_M16_C(P, 3, MI) break;     \
     }
     if (prec == 0) {
         if (mirror) { DD_M16_T(0, true) } else { DD_M16_T(0, false) }
@@ -594,6 +837,7 @@ extern "C" hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2*
         if (mirror) { DD_M16_T(2, true) } else { DD_M16_T(2, false) }
     }
 #undef DD_M16_T
+#undef DD_M16_C
 #undef DD_M16
     return hipGetLastError();
 }

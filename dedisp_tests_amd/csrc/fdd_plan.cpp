@@ -286,7 +286,8 @@ void FDDGPUPlan::prepare_tables()
 
 // Overlap-save segment length for series with nsc[0..n) outputs per DM: the
 // requested one, else the supported length (2^13 .. 2^18) with the smallest
-// summed contraction volume (chunks * (nfft/2 + 1)). The reference instead
+// contraction volume (chunk columns of the contraction kernel's tiles, padding
+// included, times nfft/2 + 1). The reference instead
 // doubles nfft from 16384 until a fifth of it covers the maximum delay
 // (src/fdd/FDDGPUPlan.cpp:756-762); the smallest volume needs fewer complex
 // multiply-accumulates for the same outputs.
@@ -296,14 +297,25 @@ size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
     const size_t dmax = (size_t)st.dmax;
     size_t nfft = st.nfft_req;
     if (!nfft) {
-        // smallest contraction volume nchunk * (nfft/2 + 1) among the supported lengths
+        // smallest contraction volume (chunk columns the kernel computes,
+        // padding included) x (nfft/2 + 1) among the supported lengths
+        const Contraction kind = contraction_kind(st.precision);
         size_t best = ~size_t(0);
         for (size_t cand = 8192; cand <= 262144; cand *= 2) {
             if (cand < dmax + 2 + 2) continue;
             const size_t good = (cand - dmax) & ~size_t(1);
             size_t chunks = 0;
             for (size_t i = 0; i < n; ++i) chunks += div_up(nsc[i], good);
-            const size_t cost = chunks * (cand / 2 + 1);
+            size_t cols = chunks;
+            if (kind == Contraction::MFMA32) {
+                cols = round_up(chunks, 32);
+            } else if (kind == Contraction::MFMA16) {
+                int ntl = 1;
+                size_t groups = 0;
+                dd_fdd_mfma16_tiles(chunks, (int)st.precision, &ntl, &groups);
+                cols = groups * (size_t)ntl * 8;
+            }
+            const size_t cost = cols * (cand / 2 + 1);
             if (cost < best) { best = cost; nfft = cand; }
         }
         if (!nfft) DD_THROW(DEDISP_UNKNOWN_ERROR);  // max delay beyond the largest segment
@@ -381,9 +393,45 @@ bool env_off(const char* name)
     return s && std::atoi(s) == 0;
 }
 
+// DEDISP_FDD_M16CFG as set (0, 1 or 2), -1 when unset.
+int m16cfg_env()
+{
+    const char* s = std::getenv("DEDISP_FDD_M16CFG");
+    if (!s) return -1;
+    const int v = std::atoi(s);
+    return (v >= 0 && v <= 2) ? v : -1;
+}
+
+// Workgroup shape of the 16 x 16 tile kernel: 2, the software-pipelined form
+// (8 waves of 2 DM tiles, 4 bin slots), for bfloat16 and float16 with the
+// delays computed in the kernel while a K-step's spectra stay within 32-bit
+// element offsets; else 0 (8 waves of 2 DM tiles, 4 slots). DEDISP_FDD_M16CFG
+// forces 0, 1 (4 waves of 4 DM tiles, 2 slots) or 2 where it applies.
+int mfma16_cfg(dedisp_amd_fdd_precision prec, size_t nchunk, size_t Xp)
+{
+    const bool pipe = prec != DEDISP_AMD_FDD_FP32 && !env_flag("DEDISP_FDD_QTABLE") &&
+                      32 * nchunk * Xp + Xp < (size_t(1) << 32);
+    const int e = m16cfg_env();
+    if (e == 0 || e == 1 || (e == 2 && pipe)) return e;
+    return pipe ? 2 : 0;
+}
+
+// Whether the 16 x 16 tile kernel pairs each bin with its mirror
+// (DEDISP_FDD_MIRROR=0/1 forces it): by default for bfloat16, whose mirror
+// phasors save the cosine, sine and hi/lo split of half the bins, and not for
+// float16 and float32, measured faster on four consecutive bins (1024 ch x
+// 2048 DMs x 23 chunks of 65536: float16 28.7 against 36.1 ms, float32 117.1
+// against 119.5 ms).
+bool mfma16_mirror(dedisp_amd_fdd_precision prec)
+{
+    const char* s = std::getenv("DEDISP_FDD_MIRROR");
+    if (s) return std::atoi(s) != 0;
+    return prec == DEDISP_AMD_FDD_BF16;
+}
+
 bool contraction_packs(const FDDState& st)
 {
-    return contraction_kind(st.precision) == Contraction::MFMA16 && !env_off("DEDISP_FDD_MIRROR") &&
+    return contraction_kind(st.precision) == Contraction::MFMA16 && mfma16_mirror(st.precision) &&
            !env_off("DEDISP_FDD_PACKED");
 }
 
@@ -408,12 +456,13 @@ void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, s
         // two bins and their mirrors
         // delays from the DM list and delay table in the kernel
         // (DEDISP_FDD_QTABLE=1: read from the phase-step table instead)
-        DD_HIP(dd_fdd_contract_mfma16((int)st.precision, !env_off("DEDISP_FDD_MIRROR"), st.d_Z.as<float2>(), Xp,
+        DD_HIP(dd_fdd_contract_mfma16((int)st.precision, mfma16_mirror(st.precision), st.d_Z.as<float2>(), Xp,
                                       st.d_q32.as<uint32_t>() + b0 * st.qpitch32, st.qpitch32, nb, nchan, nchunk,
                                       nfft, xscale, st.d_Y.as<float2>(), Yp,
                                       contraction_packs(st) ? st.d_ph.as<float2>() : nullptr,
                                       env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_dm32.as<float>() + b0,
-                                      env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_delay32.as<float>(), stream));
+                                      env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_delay32.as<float>(),
+                                      mfma16_cfg(st.precision, nchunk, Xp), stream));
         break;
     }
 }
@@ -730,7 +779,8 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
                                  (uint64_t)st.d_q32.ptr, (uint64_t)st.d_ph.ptr, (uint64_t)st.d_ph64k.ptr,
                                  (uint64_t)st.d_fwork.ptr, (uint64_t)st.d_iwork.ptr,
                                  (uint64_t)contraction_kind(st.precision), (uint64_t)contraction_packs(st),
-                                 (uint64_t)!env_off("DEDISP_FDD_MIRROR"),
+                                 (uint64_t)mfma16_mirror(st.precision), (uint64_t)(m16cfg_env() + 1),
+                                 (uint64_t)env_flag("DEDISP_FDD_QTABLE"),
                                  (uint64_t)st.d_scale.ptr, (uint64_t)st.d_delays.ptr};
     for (size_t b = 0; b < nbeams; ++b) {
         key.push_back(nsamps[b]);

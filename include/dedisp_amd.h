@@ -67,8 +67,9 @@ dedisp_error dedisp_amd_fdd_set_output(dedisp_plan plan, dedisp_amd_fdd_output m
 dedisp_error dedisp_amd_fdd_set_precision(dedisp_plan plan, dedisp_amd_fdd_precision precision);
 /* enable=1 selects the time-segmented (overlap-save) path. nfft is the
  * segment length: a power of two from 2^13 to 2^18 larger than max_delay + 3,
- * or 0 for the supported length with the smallest contraction volume
- * (chunks x (nfft/2 + 1)); the reference instead doubles nfft from 16384 until
+ * or 0 for the supported length with the smallest contraction volume (chunk
+ * columns of the contraction's tiles x (nfft/2 + 1)); the reference instead
+ * doubles nfft from 16384 until
  * a fifth of it covers max_delay (src/fdd/FDDGPUPlan.cpp:756-762). The same
  * choice picks the segments of the time-series and Fourier outputs when no
  * length is set, so any max_delay below 2^18 - 3 samples is supported. The

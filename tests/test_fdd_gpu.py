@@ -277,8 +277,10 @@ def test_fdd_host_pointers_many_dm_batches(dd):
 @pytest.mark.parametrize("nchunk_target", [1, 8, 9, 16, 23, 25, 33])
 def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nchunk_target):
     """The 16 x 16 tile contraction (default for bfloat16 / float16) with
-    mirror bins writing the inverse's packed spectrum (default) or plain bins,
-    without mirror bins, and the 32 x 32 tile kernel, against the oracle for
+    mirror bins writing the inverse's packed spectrum or plain bins, without
+    mirror bins, in each workgroup shape (the software-pipelined default, 8
+    waves, 4 waves of 2 slots), and the 32 x 32 tile kernel, against the
+    oracle for
     chunk counts that fill 1, 2 or 3 N-tiles and several chunk groups; 100
     channels pad to 128 rows."""
     p = {"bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16, "fp32-tile16": dd.FDD_FP32}[prec]
@@ -296,10 +298,18 @@ def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nch
     delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
     ref = None
     outs = {}
-    for mode, env in (("mirror", {}), ("mirror-bins", {"DEDISP_FDD_PACKED": "0"}),
-                      ("direct", {"DEDISP_FDD_MIRROR": "0"}), ("qtable", {"DEDISP_FDD_QTABLE": "1"}),
+    on = {"DEDISP_FDD_MIRROR": "1"}
+    for mode, env in (("default", {}), ("mirror", on), ("mirror-bins", {**on, "DEDISP_FDD_PACKED": "0"}),
+                      ("direct", {"DEDISP_FDD_MIRROR": "0"}), ("qtable", {**on, "DEDISP_FDD_QTABLE": "1"}),
+                      ("wg8", {**on, "DEDISP_FDD_M16CFG": "0"}), ("wg8-direct", {"DEDISP_FDD_M16CFG": "0",
+                                                                                "DEDISP_FDD_MIRROR": "0"}),
+                      ("wg4", {**on, "DEDISP_FDD_M16CFG": "1"}), ("wg4-direct", {"DEDISP_FDD_M16CFG": "1",
+                                                                                "DEDISP_FDD_MIRROR": "0"}),
+                      ("pipe", {**on, "DEDISP_FDD_M16CFG": "2"}),
+                      ("pipe-direct", {"DEDISP_FDD_M16CFG": "2", "DEDISP_FDD_MIRROR": "0"}),
                       ("tile32", {"DEDISP_FDD_MFMA32": "1"})):
-        for k in ("DEDISP_FDD_MIRROR", "DEDISP_FDD_MFMA32", "DEDISP_FDD_PACKED", "DEDISP_FDD_QTABLE"):
+        for k in ("DEDISP_FDD_MIRROR", "DEDISP_FDD_MFMA32", "DEDISP_FDD_PACKED", "DEDISP_FDD_QTABLE",
+                  "DEDISP_FDD_M16CFG"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
