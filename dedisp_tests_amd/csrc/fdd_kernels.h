@@ -76,20 +76,26 @@ hipError_t dd_fdd_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
 // points per segment (decimation in time) and a radix-R combine, through
 // work (dd_fdd_split_work(nfft, nchan*nchunk) or (nrows*nchunk) bytes).
 // ph64k: e^{+2 pi i m/65536} (m < 65536), the on-chip engine's table.
+// blocked != 0 (nfft <= 65536): X in the bin-block layout instead of rows of
+// Xp: bins k, k+1 (k even, k <= M/2) and their mirrors M-k, M-k-1 form block
+// k/2 of NBB = M/4 + 1 blocks, element (c, k, chunk) at
+// ((c*NBB + block)*nchunk + chunk)*4 + slot, slot k&1 for k <= M/2 and
+// 2 + ((M-k)&1) above (the layout dd_fdd_contract_mfma16 reads with blocked).
 hipError_t dd_fdd_forward_split(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
                                 size_t period, const float* chan_scale, const float2* ph, const float2* ph64k,
                                 size_t nfft, float2* X, size_t nchan, size_t nchunk, size_t Xp, unsigned nbits,
-                                float2* work, hipStream_t stream);
-// packed != 0: Y holds the packed half-length spectrum Zu[0..M-1] of each
-// transform (dd_fdd_contract_mfma16 with emit_packed) instead of bins 0..M.
+                                float2* work, int blocked, hipStream_t stream);
+// packed 1: Y holds the packed half-length spectrum Zu[0..M-1] of each
+// transform (dd_fdd_contract_mfma16 with ph_packed) instead of bins 0..M;
+// packed 2 (nfft <= 65536): the packed spectrum in the bin-block layout.
 hipError_t dd_fdd_inverse_split(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
                                 const float2* ph64k, size_t nfft, float* out, size_t out_stride, size_t good,
                                 size_t nsamps_out, float2* work, int packed, hipStream_t stream);
 // The chunk-table forms (dd_fdd_forward_chunks / dd_fdd_inverse_chunks) for
-// every supported nfft.
+// every supported nfft (the layouts of a launch's nchunk chunks as above).
 hipError_t dd_fdd_forward_chunks_split(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
                                        const float2* ph, const float2* ph64k, size_t nfft, float2* X, size_t nchan,
-                                       size_t Xp, unsigned nbits, float2* work, hipStream_t stream);
+                                       size_t Xp, unsigned nbits, float2* work, int blocked, hipStream_t stream);
 hipError_t dd_fdd_inverse_chunks_split(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
                                        size_t nchunk, size_t Yp, const float2* ph, const float2* ph64k, size_t nfft,
                                        size_t good, float2* work, int packed, hipStream_t stream);
@@ -141,11 +147,16 @@ hipError_t dd_fdd_contract_mfma(int prec, const float2* X, size_t Xp, const uint
 // not read.
 // cfg selects the workgroup shape: 0 = 8 waves of 2 16-row DM tiles and 4
 // bin slots; 1 = 4 waves of 4 DM tiles and 2 bin slots (half the B-operand
-// LDS reads per product, two workgroups per CU).
+// LDS reads per product, two workgroups per CU); 2 = the software-pipelined
+// form of 0 (bfloat16 / float16 with dmlist; a K-step's spectra within 32-bit
+// element offsets). blocked != 0 (cfg 2 with mirror and ph_packed, nfft <=
+// 65536): X and the packed output in the bin-block layout of
+// dd_fdd_forward_split (Xp and Yp unused).
 hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2* X, size_t Xp, const uint32_t* q,
                                   size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
                                   float xscale, float2* Y, size_t Yp, const float2* ph_packed,
-                                  const float* dmlist, const float* delay32, int cfg, hipStream_t stream);
+                                  const float* dmlist, const float* delay32, int cfg, int blocked,
+                                  hipStream_t stream);
 // N-tiles (8 chunks each) per workgroup dd_fdd_contract_mfma16 uses for
 // nchunk chunks, and the number of chunk groups.
 int dd_fdd_mfma16_tiles(size_t nchunk, int prec, int* ntl, size_t* ngroups);

@@ -152,7 +152,12 @@ struct Geo {
 // Measured on 1024 ch x 2048 DMs x 23 chunks of 65536 (bfloat16 with mirror
 // bins): 45.6 ms against 49.6 ms for cfg 0; 4 waves of 4 tiles (one wave per
 // SIMD, 512 registers) ran 56.5 ms, its waits on the LDS exposed.
-template <int PREC, int NTL, bool MIRROR, int WAVES, int MT, int NS, bool PIPE = false>
+//
+// BLK (PIPE with MIRROR and the packed output): the spectra and the packed
+// output in the bin-block layout (bins k0, k0+1, M-k0, M-k0-1 of a workgroup
+// contiguous for every chunk), so that the staging loads read and the
+// epilogue writes whole cache lines instead of 8- and 16-byte pieces.
+template <int PREC, int NTL, bool MIRROR, int WAVES, int MT, int NS, bool PIPE = false, bool BLK = false>
 __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
     const float2* __restrict__ X, size_t Xp, const uint32_t* __restrict__ q, size_t qpitch,
     uint32_t ndm, uint32_t nchan, uint32_t nchunk, uint32_t nfft, float xscale, float yscale,
@@ -160,6 +165,7 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
     const float* __restrict__ dmlist, const float* __restrict__ dtab)
 {
     static_assert(!PIPE || (PREC != 0 && NS == 4 && WAVES == 8), "pipelined form: bfloat16 / float16");
+    static_assert(!BLK || (PIPE && MIRROR), "bin-block layout: pipelined form with mirror bins");
     constexpr int NT = WAVES * 64;        // threads
     constexpr int DPB = WAVES * MT * 16;  // DMs per workgroup
     constexpr int NDIR = MIRROR ? NS / 2 : NS;  // bins whose phasors are computed
@@ -586,9 +592,27 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
         // time). Element offsets from X + kofs fit 32 bits (the launcher checks).
         constexpr int IH = (G::IPT + 1) / 2;  // items of the first half
         uint32_t xoff[G::IPT];
+        // bin blocks of the bin-block layout (M/4 + 1, the grid's nbb)
+        const uint32_t nbk = M / 4 + 1;
+        if constexpr (BLK) {
+            // item (slot s, chunk jj, channel pair cp), s fastest, then jj: a
+            // wave's loads cover 16 chunks x 4 slots, 512 contiguous bytes
 #pragma unroll
-        for (int u = 0; u < G::IPT; ++u) xoff[u] = (uint32_t)(src[u] - X);
-        const uint32_t rs32 = (uint32_t)rs;
+            for (int u = 0; u < G::IPT; ++u) {
+                const int it = tid + u * NT;
+                const int s = it & 3, jj = (it >> 2) % G::CG, cp = (it >> 2) / G::CG;
+                const uint32_t chunk = ch0 + jj;
+                xoff[u] = ((2 * cp * nbk + bblk) * nchunk + min(chunk, nchunk - 1)) * 4 + s;
+                sscale[u] = (slot_valid(s) && chunk < nchunk) ? xscale : 0.f;
+                soff[u] = s * G::SSTR + (16 * (jj / 8) + (jj % 8)) * COLB + 2 * PL::ELEM * cp;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < G::IPT; ++u) xoff[u] = (uint32_t)(src[u] - X);
+        }
+        // channel stride and K-step stride of the spectra, elements
+        const uint32_t rs32 = BLK ? nbk * nchunk * 4 : (uint32_t)rs;
+        const size_t kstep_p = BLK ? (size_t)KS * rs32 : kstep;
         float2 xh[IH][2];
         auto load_half = [&](size_t kofs, int p) __attribute__((always_inline)) {
             const float2* xb = X + kofs;
@@ -651,7 +675,7 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
             char* nbuf = xs + ((ks + 1) & 1) * BUF;
             if (u == 0) {
                 store_half(nbuf, 0);
-                load_half((size_t)((ks + 1 < nks) ? ks + 1 : ks) * kstep, 1);
+                load_half((size_t)((ks + 1 < nks) ? ks + 1 : ks) * kstep_p, 1);
             }
             if (last) store_half(nbuf, 1);
         };
@@ -669,7 +693,7 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
         __syncthreads();
         for (uint32_t ks = 0; ks < nks; ++ks) {
             const uint32_t kn = (ks + 1 < nks) ? ks + 1 : ks;
-            load_half((size_t)kn * kstep, 0);
+            load_half((size_t)kn * kstep_p, 0);
             unit(ks, 0, A0, A1);
             unit(ks, 1, A1, A0);
             if constexpr (NU == 4) {
@@ -712,6 +736,35 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
                 const uint32_t dmo = dmw + mt * 16 + row;
                 if (dmo >= ndm) continue;
                 float2* dst = Y + ((size_t)dmo * nchunk + chunk) * Yp;
+                if constexpr (BLK) {
+                    // the four packed values of the bin block, 32 contiguous bytes
+                    float2 y[NS], z[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) y[s] = make_float2(vr[s][rr] * yscale, vi[s][rr] * yscale);
+                    auto zu = [&](uint32_t kk, float2 a, float2 m) -> float2 {
+                        float2 b = make_float2(m.x, -m.y);
+                        if (kk == 0) a.y = 0.f, b.y = 0.f;
+                        const float2 e = make_float2(a.x + b.x, a.y + b.y);
+                        const float2 d = make_float2(a.x - b.x, a.y - b.y);
+                        const float2 w = zph[kk];
+                        const float2 o = make_float2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+                        return make_float2(e.x - o.y, e.y + o.x);
+                    };
+#pragma unroll
+                    for (int s = 0; s < NDIR; ++s) {
+                        const uint32_t k = k0 + s;
+                        z[s] = slot_valid(s) ? zu(k, y[s], k == M / 2 ? y[s] : y[s + NDIR]) : make_float2(0.f, 0.f);
+                    }
+#pragma unroll
+                    for (int s = NDIR; s < NS; ++s) {
+                        const uint32_t kk = M - (k0 + s - NDIR);
+                        z[s] = (slot_valid(s) && kk < M) ? zu(kk, y[s], y[s - NDIR]) : make_float2(0.f, 0.f);
+                    }
+                    float4* o = reinterpret_cast<float4*>(Y + (((size_t)dmo * (M / 4 + 1) + bblk) * nchunk + chunk) * 4);
+                    o[0] = make_float4(z[0].x, z[0].y, z[1].x, z[1].y);
+                    o[1] = make_float4(z[2].x, z[2].y, z[3].x, z[3].y);
+                    continue;
+                }
                 if constexpr (MIRROR) {
                     if (zph) {
                         // packed spectrum of the inverse transform:
@@ -786,7 +839,7 @@ extern "C" int dd_fdd_mfma16_tiles(size_t nchunk, int prec, int* ntl_out, size_t
 extern "C" hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2* X, size_t Xp, const uint32_t* q,
                                              size_t qpitch, size_t ndm, size_t nchan, size_t nchunk, size_t nfft,
                                              float xscale, float2* Y, size_t Yp, const float2* ph_packed,
-                                             const float* dmlist, const float* delay32, int cfg,
+                                             const float* dmlist, const float* delay32, int cfg, int blocked,
                                              hipStream_t stream)
 {
     if (ph_packed && !mirror) return hipErrorInvalidValue;
@@ -794,13 +847,15 @@ extern "C" hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2*
     if (!dmlist && !q) return hipErrorInvalidValue;
     if (ndm == 0 || nchunk == 0) return hipSuccess;
     // layout preconditions of the vector loads and stores
-    if ((!dmlist && (qpitch % 32 || qpitch < (nchan + 31) / 32 * 32)) || Yp % 2 || nfft < 64 ||
-        nfft > (size_t(1) << 18) || (nfft & (nfft - 1)) || Xp < nfft / 2 + 1)
+    if ((!dmlist && (qpitch % 32 || qpitch < (nchan + 31) / 32 * 32)) || (!blocked && Yp % 2) || nfft < 64 ||
+        nfft > (size_t(1) << 18) || (nfft & (nfft - 1)) || (!blocked && Xp < nfft / 2 + 1))
         return hipErrorInvalidValue;
     if (prec != 0 && prec != 1 && prec != 2) return hipErrorInvalidValue;
     if (cfg < 0 || cfg > 2 || (cfg == 2 && (prec == 0 || !dmlist))) return hipErrorInvalidValue;
+    if (blocked && (cfg != 2 || !mirror || !ph_packed || nfft > 65536)) return hipErrorInvalidValue;
     // the pipelined form addresses the spectra of a K-step by 32-bit element offsets
-    if (cfg == 2 && (size_t)KS * nchunk * Xp + Xp >= (size_t(1) << 32)) return hipErrorInvalidValue;
+    const size_t rowp = blocked ? nfft / 2 + 4 : Xp;
+    if (cfg == 2 && (size_t)KS * nchunk * rowp + rowp >= (size_t(1) << 32)) return hipErrorInvalidValue;
     const size_t M = nfft / 2;
     // cfg 0: 8 waves x 2 DM tiles, 4 bin slots; cfg 1: 4 waves x 4 DM tiles, 2
     // bin slots; cfg 2: pipelined, 8 waves x 2 DM tiles, 4 bin slots
@@ -815,14 +870,15 @@ extern "C" hipError_t dd_fdd_contract_mfma16(int prec, int mirror, const float2*
     const size_t nblocks = (size_t)nbb * ndb * ngroups;
     if (nblocks >= (size_t(1) << 31)) return hipErrorInvalidValue;
     const float yscale = 1.0f / xscale;
-#define DD_M16(P, T, MI, W, MT_, NS_, PI)                                                                        \
-    hipLaunchKernelGGL((fdd_mfma16_k<P, T, MI, W, MT_, NS_, PI>), dim3((unsigned)nblocks), dim3(W * 64), 0, stream, \
-                       X, Xp, q, qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, (uint32_t)nfft, xscale, \
-                       yscale, Y, Yp, nbb, ndb, ph_packed, dmlist, delay32)
+#define DD_M16(P, T, MI, W, MT_, NS_, PI, BL)                                                                    \
+    hipLaunchKernelGGL((fdd_mfma16_k<P, T, MI, W, MT_, NS_, PI, BL>), dim3((unsigned)nblocks), dim3(W * 64), 0,      \
+                       stream, X, Xp, q, qpitch, (uint32_t)ndm, (uint32_t)nchan, (uint32_t)nchunk, (uint32_t)nfft,   \
+                       xscale, yscale, Y, Yp, nbb, ndb, ph_packed, dmlist, delay32)
 #define DD_M16_C(P, T, MI)                                                               \
-    if (cfg == 1) DD_M16(P, T, MI, 4, 4, 2, false);                                      \
-    else if (cfg == 2) DD_M16(P, T, MI, 8, 2, 4, P != 0);                                \
-    else DD_M16(P, T, MI, 8, 2, 4, false);
+    if (cfg == 1) DD_M16(P, T, MI, 4, 4, 2, false, false);                               \
+    else if (cfg == 2 && blocked) DD_M16(P, T, MI, 8, 2, 4, P != 0, P != 0 && MI);       \
+    else if (cfg == 2) DD_M16(P, T, MI, 8, 2, 4, P != 0, false);                         \
+    else DD_M16(P, T, MI, 8, 2, 4, false, false);
 #define DD_M16_T(P, MI)                    \
     switch (ntl) {                         \
     case 1: DD_M16_C(P, 1, MI) break;      \

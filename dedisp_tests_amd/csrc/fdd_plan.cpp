@@ -417,16 +417,16 @@ int mfma16_cfg(dedisp_amd_fdd_precision prec, size_t nchunk, size_t Xp)
 }
 
 // Whether the 16 x 16 tile kernel pairs each bin with its mirror
-// (DEDISP_FDD_MIRROR=0/1 forces it): by default for bfloat16, whose mirror
-// phasors save the cosine, sine and hi/lo split of half the bins, and not for
-// float16 and float32, measured faster on four consecutive bins (1024 ch x
-// 2048 DMs x 23 chunks of 65536: float16 28.7 against 36.1 ms, float32 117.1
-// against 119.5 ms).
+// (DEDISP_FDD_MIRROR=0/1 forces it): by default for bfloat16 and float16,
+// whose mirror phasors save a cosine, a sine (and the hi/lo split) for half
+// the bins and which then run in the bin-block layout (1024 ch x 2048 DMs x
+// 23 chunks of 65536: float16 21.2 ms against 28.7 ms on four consecutive
+// bins); not for float32 (117.1 ms on consecutive bins against 119.5 ms).
 bool mfma16_mirror(dedisp_amd_fdd_precision prec)
 {
     const char* s = std::getenv("DEDISP_FDD_MIRROR");
     if (s) return std::atoi(s) != 0;
-    return prec == DEDISP_AMD_FDD_BF16;
+    return prec != DEDISP_AMD_FDD_FP32;
 }
 
 bool contraction_packs(const FDDState& st)
@@ -435,11 +435,30 @@ bool contraction_packs(const FDDState& st)
            !env_off("DEDISP_FDD_PACKED");
 }
 
+// Whether the spectra and the packed output use the bin-block layout (the
+// pipelined float16 / bfloat16 contraction with mirror bins and the packed
+// output, nfft <= 65536): a workgroup of the contraction then reads and
+// writes whole cache lines (DEDISP_FDD_BLOCKED=0 keeps rows of Xp / Yp).
+bool blocked_layout(const FDDState& st, size_t nfft, size_t nchunk, size_t Xp)
+{
+    return contraction_packs(st) && mfma16_cfg(st.precision, nchunk, Xp) == 2 && nfft <= 65536 &&
+           !env_off("DEDISP_FDD_BLOCKED");
+}
+
+// Elements of one channel's spectra: rows of Xp per chunk, or the bin blocks.
+size_t chan_elems(bool blocked, size_t nfft, size_t nchunk, size_t Xp)
+{
+    return blocked ? (nfft / 8 + 1) * 4 * nchunk : nchunk * Xp;
+}
+
+// The inverse transforms' input form: 0 bins, 1 packed rows, 2 packed bin blocks.
+int inverse_form(const FDDState& st, bool blocked) { return blocked ? 2 : (contraction_packs(st) ? 1 : 0); }
+
 // Contraction of DMs b0 .. b0+nb of the plan's tables over the spectra in
 // st.d_Z ([padded channels][nchunk][Xp]) into st.d_Y ([nb][nchunk][Yp]):
 // dedispersed bins 0..M, or the packed spectrum when contraction_packs.
 void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, size_t nfft, size_t Xp, size_t Yp,
-              float xscale, hipStream_t stream)
+              float xscale, bool blocked, hipStream_t stream)
 {
     switch (contraction_kind(st.precision)) {
     case Contraction::VALU:
@@ -462,7 +481,7 @@ void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, s
                                       contraction_packs(st) ? st.d_ph.as<float2>() : nullptr,
                                       env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_dm32.as<float>() + b0,
                                       env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_delay32.as<float>(),
-                                      mfma16_cfg(st.precision, nchunk, Xp), stream));
+                                      mfma16_cfg(st.precision, nchunk, Xp), blocked ? 1 : 0, stream));
         break;
     }
 }
@@ -571,10 +590,11 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     st.d_rows.reserve(nchan * row_words * 4);
     // channel rows padded to a multiple of 32 with zeros (matrix-core staging)
     const size_t nchan_pad = padded_channels(nchan);
+    const bool blocked = blocked_layout(st, nfft, nchunk, Xp);
+    const size_t zrow = chan_elems(blocked, nfft, nchunk, Xp);
     st.d_Z.reserve(nchan_pad * nchunk * Xp * 8);
     if (nchan_pad > nchan)
-        DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * nchunk * Xp, 0,
-                              (nchan_pad - nchan) * nchunk * Xp * 8, stream));
+        DD_HIP(hipMemsetAsync(st.d_Z.as<float2>() + nchan * zrow, 0, (nchan_pad - nchan) * zrow * 8, stream));
     ctx.timer.begin(StageTimer::UNPACK, stream);
     DD_HIP(dd_tdd_transpose(d_in, nsamps, in_row, (unsigned)in_nbits, nchan, st.d_rows.as<uint32_t>(),
                             row_words, stream));
@@ -583,7 +603,8 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     ctx.timer.begin(StageTimer::FFT, stream);
     DD_HIP(dd_fdd_forward_split(st.d_rows.as<uint8_t>(), row_words * 4, good, nsamps, fourier ? Nf : 0,
                                 st.d_scale.as<float>(), ph, st.d_ph64k.as<float2>(), nfft, st.d_Z.as<float2>(),
-                                nchan, nchunk, Xp, (unsigned)in_nbits, st.d_fwork.as<float2>(), stream));
+                                nchan, nchunk, Xp, (unsigned)in_nbits, st.d_fwork.as<float2>(), blocked ? 1 : 0,
+                                stream));
     ctx.timer.end(StageTimer::FFT, stream);
 
     // ---- per DM batch: contraction, inverse transforms ----
@@ -619,17 +640,17 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
     for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
         const size_t nb = std::min(dm_batch, ndm - b0);
         ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-        contract(st, nchan, b0, nb, nchunk, nfft, Xp, Yp, xscale, stream);
+        contract(st, nchan, b0, nb, nchunk, nfft, Xp, Yp, xscale, blocked, stream);
         ctx.timer.end(StageTimer::DEDISPERSE, stream);
         ctx.timer.begin(StageTimer::IFFT, stream);
         if (!fourier) {
             DD_HIP(dd_fdd_inverse_split(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, st.d_ph64k.as<float2>(), nfft,
                                         d_out + b0 * nsc, nsc, good, nsc, st.d_iwork.as<float2>(),
-                                        contraction_packs(st), stream));
+                                        inverse_form(st, blocked), stream));
         } else {
             DD_HIP(dd_fdd_inverse_split(st.d_Y.as<float2>(), nb, nchunk, Yp, ph, st.d_ph64k.as<float2>(), nfft,
                                         st.d_S.as<float>(), Nf, good, Nf, st.d_iwork.as<float2>(),
-                                        contraction_packs(st), stream));
+                                        inverse_form(st, blocked), stream));
             DD_HIP(dd_fdd_rfft_rows(st.d_S.as<float>(), Nf, nb, Nf, st.d_ph16k.as<float2>(),
                                     st.d_twL.as<float2>(), st.d_twN.as<float2>(),
                                     st.d_work.as<float2>(),
@@ -780,7 +801,7 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
                                  (uint64_t)st.d_fwork.ptr, (uint64_t)st.d_iwork.ptr,
                                  (uint64_t)contraction_kind(st.precision), (uint64_t)contraction_packs(st),
                                  (uint64_t)mfma16_mirror(st.precision), (uint64_t)(m16cfg_env() + 1),
-                                 (uint64_t)env_flag("DEDISP_FDD_QTABLE"),
+                                 (uint64_t)env_flag("DEDISP_FDD_QTABLE"), (uint64_t)env_off("DEDISP_FDD_BLOCKED"),
                                  (uint64_t)st.d_scale.ptr, (uint64_t)st.d_delays.ptr};
     for (size_t b = 0; b < nbeams; ++b) {
         key.push_back(nsamps[b]);
@@ -835,8 +856,10 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
     // the kernels of one group (no copies: they may be captured)
     auto group_kernels = [&](const Group& g, bool timed) {
         const size_t nc = g.nchunk;
+        const bool blocked = blocked_layout(st, nfft, nc, Xp);
+        const size_t zrow = chan_elems(blocked, nfft, nc, Xp);
         if (nchan_pad > nchan)
-            DD_HIP(dd_zero_fill(st.d_Z.as<float2>() + nchan * nc * Xp, (nchan_pad - nchan) * nc * Xp * 8, stream));
+            DD_HIP(dd_zero_fill(st.d_Z.as<float2>() + nchan * zrow, (nchan_pad - nchan) * zrow * 8, stream));
         if (timed) ctx.timer.begin(StageTimer::UNPACK, stream);
         for (size_t b = g.b0; b < g.b1; ++b) {
             if (!nch[b]) continue;
@@ -848,17 +871,17 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
         if (timed) ctx.timer.begin(StageTimer::FFT, stream);
         DD_HIP(dd_fdd_forward_chunks_split(d_tab + g.chunk0, nc, st.d_scale.as<float>(), ph,
                                            st.d_ph64k.as<float2>(), nfft, st.d_Z.as<float2>(), nchan, Xp,
-                                           (unsigned)in_nbits, st.d_fwork.as<float2>(), stream));
+                                           (unsigned)in_nbits, st.d_fwork.as<float2>(), blocked ? 1 : 0, stream));
         if (timed) ctx.timer.end(StageTimer::FFT, stream);
         for (size_t b0 = 0; b0 < ndm; b0 += dm_batch) {
             const size_t nb = std::min(dm_batch, ndm - b0);
             if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
-            contract(st, nchan, b0, nb, nc, nfft, Xp, Yp, xscale, stream);
+            contract(st, nchan, b0, nb, nc, nfft, Xp, Yp, xscale, blocked, stream);
             if (timed) ctx.timer.end(StageTimer::DEDISPERSE, stream);
             if (timed) ctx.timer.begin(StageTimer::IFFT, stream);
             DD_HIP(dd_fdd_inverse_chunks_split(st.d_Y.as<float2>(), nb, b0, d_tab + g.chunk0, nc, Yp, ph,
                                                st.d_ph64k.as<float2>(), nfft, good, st.d_iwork.as<float2>(),
-                                               contraction_packs(st), stream));
+                                               inverse_form(st, blocked), stream));
             if (timed) ctx.timer.end(StageTimer::IFFT, stream);
         }
     };

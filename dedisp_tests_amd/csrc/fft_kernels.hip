@@ -81,6 +81,31 @@ struct FftShape {
     static constexpr int LDS_FLOATS = X1 > X2 ? X1 : X2;
 };
 
+// Bin-block layout of the real spectra (the float16 / bfloat16 matrix-core
+// contraction with mirror bins reads and writes it): bins k and k+1 (k even,
+// k <= M/2) and their mirrors M-k, M-k-1 form block k/2, the four values of a
+// block for every chunk contiguous, so a workgroup of the contraction reads
+// and writes whole cache lines. Element (row, k, chunk) of a launch of nchunk
+// chunks sits at ((row*NBB + block)*nchunk + chunk)*4 + slot, NBB = M/4 + 1:
+// slot k&1 of block k/2 for k <= M/2, else slot 2 + (j&1) of block j/2 with
+// j = M - k.
+__device__ __forceinline__ uint32_t blk_offset(uint32_t k, uint32_t M, uint32_t bstride)
+{
+    if (k <= M / 2) return (k >> 1) * bstride + (k & 1);
+    const uint32_t j = M - k;
+    return (j >> 1) * bstride + 2 + (j & 1);
+}
+
+// Workgroup order across the 8 XCDs: consecutive ids (bid) go to different
+// XCDs; L gives consecutive values to the workgroups of one XCD, so those
+// that share cache lines (neighbouring chunks in the bin-block layout) run
+// together in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_linear(uint32_t bid, uint32_t nb)
+{
+    const uint32_t xcd = bid & 7u, per = nb >> 3, rem = nb & 7u;
+    return xcd * per + min(xcd, rem) + (bid >> 3);
+}
+
 // Twiddle w^j for j = 0..R-1 given w = base: reseeded from the table every 8
 // steps (tw_at(j) returns the exact table value for j multiple of 8), so each
 // value is at most 7 complex products away from a correctly rounded one.
@@ -228,7 +253,9 @@ __device__ __forceinline__ void fft3(const float2* __restrict__ ph, float* lds, 
 // registers.
 // TAB: the chunk's rows, first sample and valid length come from tab[chunk]
 // (the chunks of several beams in one launch) instead of rows/chunk_step/nvalid.
-template <int A, int B, int C, bool TAB = false, int NT = FftShape<A, B, C>::T>
+// BLK: X in the bin-block layout (one-dimensional grid in XCD order, chunk
+// fastest) instead of rows of Xp.
+template <int A, int B, int C, bool TAB = false, bool BLK = false, int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_forward_k(
     const uint8_t* __restrict__ rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
     size_t period, const float* __restrict__ chan_scale, const float2* __restrict__ ph,
@@ -239,7 +266,12 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     static_assert(S::T3 == NT, "pass 3 uses every thread");
     constexpr int M = S::M, T3 = S::T3;
     __shared__ float lds[S::LDS_FLOATS];
-    const size_t chunk = blockIdx.x, c = blockIdx.y;
+    size_t chunk = blockIdx.x, c = blockIdx.y;
+    if constexpr (BLK) {
+        const uint32_t l = xcd_linear(blockIdx.x, gridDim.x);
+        chunk = l % nchunk;
+        c = l / nchunk;
+    }
     size_t t0 = chunk * chunk_step;
     if constexpr (TAB) {
         const dd_fdd_chunk e = tab[chunk];
@@ -251,6 +283,11 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     const uint8_t* src = rows + c * row_bytes;
     const float s = chan_scale[c];
     float2* dst = X + (c * nchunk + chunk) * Xp;
+    // bin k of this transform
+    constexpr uint32_t NBB = M / 4 + 1;
+    float2* const dstb = X + (c * NBB * nchunk + chunk) * 4;
+    const uint32_t bstride = (uint32_t)nchunk * 4u;
+    auto bin = [&](uint32_t k) -> float2* { return BLK ? dstb + blk_offset(k, M, bstride) : dst + k; };
     // x[t] = (u8 - 127.5) * scale (src/fdd/unpack/unpack_kernel.cuh:60) for
     // t < nvalid, zero beyond (the zero padding of src/fdd/FDDGPUPlan.cpp:476-483);
     // with period > 0 the series is read periodically (t mod period). Sub-byte
@@ -309,22 +346,35 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     };
     // point k = tq + T3*e has its mirror M - k at element A-1-e of the partner
     // lane (one swap of that register serves both lanes)
+    // BLK: the upper lane computes its element A-1-e in step e instead of e,
+    // so that lanes i, i+1, i+32, i+33 store the four values of a bin block
+    // (k, k+1 and their mirrors) in one instruction; each lane then sends the
+    // partner its own operand.
 #pragma unroll
     for (int e = 0; e < A; ++e) {
-        const float2 own = q[A - 1 - e];
-        float2 mz = make_float2(other_half(own.x), other_half(own.y));
-        if (selfh) mz = own;
+        const bool rev = BLK && hi;
+        const int ee = rev ? A - 1 - e : e;
+        float2 a, mz;
+        if constexpr (BLK) {
+            a = make_float2(rev ? q[A - 1 - e].x : q[e].x, rev ? q[A - 1 - e].y : q[e].y);
+            mz = make_float2(other_half(a.x), other_half(a.y));
+            if (selfh) mz = q[e];
+        } else {
+            const float2 own = q[A - 1 - e];
+            mz = make_float2(other_half(own.x), other_half(own.y));
+            if (selfh) mz = own;
+            a = q[e];
+        }
         if (self0) mz = q[(A - e) % A];
-        const int k = tq + T3 * e;
-        const float2 a = q[e];
+        const int k = tq + T3 * ee;
         const float2 pk = ph[k];  // W^k = conj(pk)
         const float ux = a.x + mz.x, uy = a.y - mz.y;  // Z[k] + conj Z[M-k]
         const float vx = a.x - mz.x, vy = a.y + mz.y;  // Z[k] - conj Z[M-k]
         // -i/2 W^k v = 1/2 (Im(W^k v), -Re(W^k v))
         const float wr = vx * pk.x + vy * pk.y, wi = vy * pk.x - vx * pk.y;
-        dst[k] = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
+        *bin(k) = make_float2(0.5f * (ux + wi), 0.5f * (uy - wr));
     }
-    if (self0) dst[M] = make_float2(q[0].x - q[0].y, 0.f);  // X[M] = Re Z[0] - Im Z[0]
+    if (self0) *bin(M) = make_float2(q[0].x - q[0].y, 0.f);  // X[M] = Re Z[0] - Im Z[0]
 }
 
 // Inverse: DM row j, chunk i of Y (M+1 bins); writes the first `good`
@@ -341,7 +391,10 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
 // output sample and output length come from tab[chunk].
 // PACKED: Y already holds the packed spectrum Zu[0..M-1] (the contraction's
 // mirror form writes it), so pass 1 loads it as it is.
-template <int A, int B, int C, bool TAB = false, bool PACKED = false, int NT = FftShape<A, B, C>::T>
+// BLK (with PACKED): Zu in the bin-block layout (one-dimensional grid in XCD
+// order, chunk fastest).
+template <int A, int B, int C, bool TAB = false, bool PACKED = false, bool BLK = false,
+          int NT = FftShape<A, B, C>::T>
 __global__ __launch_bounds__(NT) void fdd_inverse_k(
     const float2* __restrict__ Y, size_t nchunk, size_t Yp, const float2* __restrict__ ph,
     float* __restrict__ out, size_t out_stride, size_t good, size_t nsamps_out,
@@ -351,8 +404,14 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     constexpr int M = S::M, T1 = S::T1;
     static_assert(T1 % 64 == 0 && C % 2 == 0, "pass-1 threads fill whole waves");
     static_assert(S::LDS_FLOATS >= 2 * C, "index 0 stages its points");
+    static_assert(!BLK || PACKED, "the bin-block layout holds the packed spectrum");
     __shared__ float lds[S::LDS_FLOATS];
-    const size_t chunk = blockIdx.x, row = blockIdx.y;
+    size_t chunk = blockIdx.x, row = blockIdx.y;
+    if constexpr (BLK) {
+        const uint32_t l = xcd_linear(blockIdx.x, gridDim.x);
+        chunk = l % nchunk;
+        row = l / nchunk;
+    }
     const float2* src = Y + (row * nchunk + chunk) * Yp;
     size_t t0 = chunk * good;
     if constexpr (TAB) {
@@ -366,7 +425,32 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     const float inv_n = 1.0f / (float)(2 * M);
     const int t = threadIdx.x;
     float2 q[A];
-    if constexpr (PACKED) {
+    if constexpr (PACKED && BLK) {
+        constexpr uint32_t NBB = M / 4 + 1;
+        const float2* srcb = Y + (row * NBB * nchunk + chunk) * 4;
+        const uint32_t bstride = (uint32_t)nchunk * 4u;
+        // pass-1 index tp on lane i, its mirror T1 - tp on lane i + 32; the
+        // upper lane loads its element C-1-n2 in step n2 (the mirrors of the
+        // lower lane's points), so that one load instruction reads the four
+        // values of a bin block, and reverses its elements before pass 1
+        const int hi = (t >> 5) & 1;
+        const int base = (t >> 6) * 32 + (t & 31);
+        const int tp = (t < T1) ? (hi ? (base ? T1 - base : T1 / 2) : base) : t;
+        auto zload = [&](int n) -> float2 {
+            const int m = hi ? 2 * tp + T1 * (C - 1) - n : n;
+            return srcb[blk_offset((uint32_t)m, M, bstride)];
+        };
+        auto reverse = [&](float2 (&v)[C], bool act) {
+            if (!act) return;
+#pragma unroll
+            for (int n2 = 0; n2 < C / 2; ++n2) {
+                const float2 a = v[n2], b = v[C - 1 - n2];
+                v[n2] = make_float2(hi ? b.x : a.x, hi ? b.y : a.y);
+                v[C - 1 - n2] = make_float2(hi ? a.x : b.x, hi ? a.y : b.y);
+            }
+        };
+        fft3_core<A, B, C, +1>(ph, lds, zload, q, reverse, tp);
+    } else if constexpr (PACKED) {
         auto zload = [&](int k) -> float2 { return src[(uint32_t)k]; };
         fft3_core<A, B, C, +1>(ph, lds, zload, q);
     } else {
@@ -440,48 +524,82 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     }
 }
 
+// Grid of a launch over nchunk x nrows transforms: two-dimensional, or (the
+// bin-block layout) one-dimensional in XCD order.
+inline bool blk_grid(bool blk, size_t nchunk, size_t nrows, dim3& grid)
+{
+    if (!blk) {
+        grid = dim3((unsigned)nchunk, (unsigned)nrows);
+        return nchunk < (size_t(1) << 31) && nrows < 65536;
+    }
+    grid = dim3((unsigned)(nchunk * nrows));
+    return nchunk * nrows < (size_t(1) << 31);
+}
+
 template <int A, int B, int C>
 hipError_t launch_forward(const uint8_t* rows, size_t row_bytes, size_t chunk_step, size_t nvalid,
                           size_t period, const float* chan_scale, const float2* ph, float2* X,
-                          size_t nchan, size_t nchunk, size_t Xp, unsigned nbits, hipStream_t stream)
+                          size_t nchan, size_t nchunk, size_t Xp, unsigned nbits, hipStream_t stream,
+                          bool blk = false)
 {
-    if (Xp < (size_t)FftShape<A, B, C>::M + 1) return hipErrorInvalidValue;
+    if (!blk && Xp < (size_t)FftShape<A, B, C>::M + 1) return hipErrorInvalidValue;
     if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return hipErrorInvalidValue;
     if (nbits != 8 && (row_bytes % 4)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((fdd_forward_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nchan),
-                       dim3(FftShape<A, B, C>::T), 0, stream, rows, row_bytes, chunk_step, nvalid,
-                       period, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits,
-                       (const dd_fdd_chunk*)nullptr);
+    if (!nchunk || !nchan) return hipSuccess;
+    dim3 grid;
+    if (!blk_grid(blk, nchunk, nchan, grid)) return hipErrorInvalidValue;
+    if (blk)
+        hipLaunchKernelGGL((fdd_forward_k<A, B, C, false, true>), grid, dim3(FftShape<A, B, C>::T), 0, stream,
+                           rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, X, nchunk, Xp,
+                           (uint32_t)nbits, (const dd_fdd_chunk*)nullptr);
+    else
+        hipLaunchKernelGGL((fdd_forward_k<A, B, C>), grid, dim3(FftShape<A, B, C>::T), 0, stream, rows,
+                           row_bytes, chunk_step, nvalid, period, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits,
+                           (const dd_fdd_chunk*)nullptr);
     return hipGetLastError();
 }
 
 template <int A, int B, int C>
 hipError_t launch_forward_chunks(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
                                  const float2* ph, float2* X, size_t nchan, size_t Xp, unsigned nbits,
-                                 hipStream_t stream)
+                                 hipStream_t stream, bool blk = false)
 {
-    if (Xp < (size_t)FftShape<A, B, C>::M + 1 || !chunks) return hipErrorInvalidValue;
+    if ((!blk && Xp < (size_t)FftShape<A, B, C>::M + 1) || !chunks) return hipErrorInvalidValue;
     if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8) return hipErrorInvalidValue;
     if (!nchunk || !nchan) return hipSuccess;
-    hipLaunchKernelGGL((fdd_forward_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nchan),
-                       dim3(FftShape<A, B, C>::T), 0, stream, (const uint8_t*)nullptr, (size_t)0,
-                       (size_t)0, (size_t)0, (size_t)0, chan_scale, ph, X, nchunk, Xp, (uint32_t)nbits,
-                       chunks);
+    dim3 grid;
+    if (!blk_grid(blk, nchunk, nchan, grid)) return hipErrorInvalidValue;
+    if (blk)
+        hipLaunchKernelGGL((fdd_forward_k<A, B, C, true, true>), grid, dim3(FftShape<A, B, C>::T), 0, stream,
+                           (const uint8_t*)nullptr, (size_t)0, (size_t)0, (size_t)0, (size_t)0, chan_scale, ph,
+                           X, nchunk, Xp, (uint32_t)nbits, chunks);
+    else
+        hipLaunchKernelGGL((fdd_forward_k<A, B, C, true>), grid, dim3(FftShape<A, B, C>::T), 0, stream,
+                           (const uint8_t*)nullptr, (size_t)0, (size_t)0, (size_t)0, (size_t)0, chan_scale, ph,
+                           X, nchunk, Xp, (uint32_t)nbits, chunks);
     return hipGetLastError();
 }
 
+// packed: 0 bins 0..M in rows of Yp, 1 the packed spectrum in rows of Yp,
+// 2 the packed spectrum in the bin-block layout.
 template <int A, int B, int C>
 hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Yp, const float2* ph,
                           float* out, size_t out_stride, size_t good, size_t nsamps_out,
-                          hipStream_t stream, bool packed = false)
+                          hipStream_t stream, int packed = 0)
 {
-    if (packed)
-        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, false, true>), dim3((unsigned)nchunk, (unsigned)nrows),
-                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
-                           good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
+    if (packed < 0 || packed > 2) return hipErrorInvalidValue;
+    if (!nchunk || !nrows) return hipSuccess;
+    dim3 grid;
+    if (!blk_grid(packed == 2, nchunk, nrows, grid)) return hipErrorInvalidValue;
+    const dim3 block(FftShape<A, B, C>::T);
+    if (packed == 2)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, false, true, true>), grid, block, 0, stream, Y, nchunk, Yp, ph,
+                           out, out_stride, good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
+    else if (packed)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, false, true>), grid, block, 0, stream, Y, nchunk, Yp, ph, out,
+                           out_stride, good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
     else
-        hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), dim3((unsigned)nchunk, (unsigned)nrows),
-                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, out, out_stride,
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C>), grid, block, 0, stream, Y, nchunk, Yp, ph, out, out_stride,
                            good, nsamps_out, (const dd_fdd_chunk*)nullptr, (size_t)0);
     return hipGetLastError();
 }
@@ -489,18 +607,22 @@ hipError_t launch_inverse(const float2* Y, size_t nrows, size_t nchunk, size_t Y
 template <int A, int B, int C>
 hipError_t launch_inverse_chunks(const float2* Y, size_t nrows, size_t row0, const dd_fdd_chunk* chunks,
                                  size_t nchunk, size_t Yp, const float2* ph, size_t good,
-                                 hipStream_t stream, bool packed = false)
+                                 hipStream_t stream, int packed = 0)
 {
-    if (!chunks) return hipErrorInvalidValue;
+    if (!chunks || packed < 0 || packed > 2) return hipErrorInvalidValue;
     if (!nchunk || !nrows) return hipSuccess;
-    if (packed)
-        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true, true>), dim3((unsigned)nchunk, (unsigned)nrows),
-                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
-                           (size_t)0, good, (size_t)0, chunks, row0);
+    dim3 grid;
+    if (!blk_grid(packed == 2, nchunk, nrows, grid)) return hipErrorInvalidValue;
+    const dim3 block(FftShape<A, B, C>::T);
+    if (packed == 2)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true, true, true>), grid, block, 0, stream, Y, nchunk, Yp, ph,
+                           (float*)nullptr, (size_t)0, good, (size_t)0, chunks, row0);
+    else if (packed)
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true, true>), grid, block, 0, stream, Y, nchunk, Yp, ph,
+                           (float*)nullptr, (size_t)0, good, (size_t)0, chunks, row0);
     else
-        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true>), dim3((unsigned)nchunk, (unsigned)nrows),
-                           dim3(FftShape<A, B, C>::T), 0, stream, Y, nchunk, Yp, ph, (float*)nullptr,
-                           (size_t)0, good, (size_t)0, chunks, row0);
+        hipLaunchKernelGGL((fdd_inverse_k<A, B, C, true>), grid, block, 0, stream, Y, nchunk, Yp, ph,
+                           (float*)nullptr, (size_t)0, good, (size_t)0, chunks, row0);
     return hipGetLastError();
 }
 
@@ -1031,8 +1153,19 @@ extern "C" hipError_t dd_fdd_forward_split(const uint8_t* rows, size_t row_bytes
                                            size_t period, const float* chan_scale, const float2* ph,
                                            const float2* ph64k, size_t nfft, float2* X, size_t nchan,
                                            size_t nchunk, size_t Xp, unsigned nbits, float2* work,
-                                           hipStream_t stream)
+                                           int blocked, hipStream_t stream)
 {
+    if (blocked) {
+#define DD_FWD(A, B, C) launch_forward<A, B, C>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, X, nchan, nchunk, Xp, nbits, stream, true)
+        switch (nfft) {
+        case 8192:  return DD_FWD(16, 16, 16);
+        case 16384: return DD_FWD(16, 16, 32);
+        case 32768: return DD_FWD(16, 32, 32);
+        case 65536: return DD_FWD(32, 32, 32);
+#undef DD_FWD
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (nfft) {
     case 131072:
         return launch_forward_split<2>(rows, row_bytes, chunk_step, nvalid, period, chan_scale, ph, ph64k, X, nchan,
@@ -1049,9 +1182,20 @@ extern "C" hipError_t dd_fdd_forward_split(const uint8_t* rows, size_t row_bytes
 extern "C" hipError_t dd_fdd_forward_chunks_split(const dd_fdd_chunk* chunks, size_t nchunk, const float* chan_scale,
                                                   const float2* ph, const float2* ph64k, size_t nfft, float2* X,
                                                   size_t nchan, size_t Xp, unsigned nbits, float2* work,
-                                                  hipStream_t stream)
+                                                  int blocked, hipStream_t stream)
 {
     if (!chunks) return hipErrorInvalidValue;
+    if (blocked) {
+#define DD_FWD(A, B, C) launch_forward_chunks<A, B, C>(chunks, nchunk, chan_scale, ph, X, nchan, Xp, nbits, stream, true)
+        switch (nfft) {
+        case 8192:  return DD_FWD(16, 16, 16);
+        case 16384: return DD_FWD(16, 16, 32);
+        case 32768: return DD_FWD(16, 32, 32);
+        case 65536: return DD_FWD(32, 32, 32);
+#undef DD_FWD
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (nfft) {
     case 131072:
         return launch_forward_split<2>(nullptr, 0, 0, 0, 0, chan_scale, ph, ph64k, X, nchan, nchunk, Xp, nbits, work,
@@ -1069,7 +1213,7 @@ extern "C" hipError_t dd_fdd_inverse_chunks_split(const float2* Y, size_t nrows,
                                                   const float2* ph, const float2* ph64k, size_t nfft, size_t good,
                                                   float2* work, int packed, hipStream_t stream)
 {
-    if (!chunks) return hipErrorInvalidValue;
+    if (!chunks || packed < 0 || packed > 2 || (packed == 2 && nfft > 65536)) return hipErrorInvalidValue;
     switch (nfft) {
     case 8192:  return launch_inverse_chunks<16, 16, 16>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
     case 16384: return launch_inverse_chunks<16, 16, 32>(Y, nrows, row0, chunks, nchunk, Yp, ph, good, stream, packed);
@@ -1091,6 +1235,7 @@ extern "C" hipError_t dd_fdd_inverse_split(const float2* Y, size_t nrows, size_t
                                            size_t good, size_t nsamps_out, float2* work, int packed,
                                            hipStream_t stream)
 {
+    if (packed < 0 || packed > 2 || (packed == 2 && nfft > 65536)) return hipErrorInvalidValue;
     switch (nfft) {
     case 8192:
         return launch_inverse<16, 16, 16>(Y, nrows, nchunk, Yp, ph, out, out_stride, good, nsamps_out, stream, packed);

@@ -277,7 +277,8 @@ def test_fdd_host_pointers_many_dm_batches(dd):
 @pytest.mark.parametrize("nchunk_target", [1, 8, 9, 16, 23, 25, 33])
 def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nchunk_target):
     """The 16 x 16 tile contraction (default for bfloat16 / float16) with
-    mirror bins writing the inverse's packed spectrum or plain bins, without
+    mirror bins writing the inverse's packed spectrum (in the bin-block layout
+    or in rows) or plain bins, without
     mirror bins, in each workgroup shape (the software-pipelined default, 8
     waves, 4 waves of 2 slots), and the 32 x 32 tile kernel, against the
     oracle for
@@ -299,7 +300,8 @@ def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nch
     ref = None
     outs = {}
     on = {"DEDISP_FDD_MIRROR": "1"}
-    for mode, env in (("default", {}), ("mirror", on), ("mirror-bins", {**on, "DEDISP_FDD_PACKED": "0"}),
+    for mode, env in (("default", {}), ("mirror", on), ("mirror-rows", {**on, "DEDISP_FDD_BLOCKED": "0"}),
+                      ("mirror-bins", {**on, "DEDISP_FDD_PACKED": "0"}),
                       ("direct", {"DEDISP_FDD_MIRROR": "0"}), ("qtable", {**on, "DEDISP_FDD_QTABLE": "1"}),
                       ("wg8", {**on, "DEDISP_FDD_M16CFG": "0"}), ("wg8-direct", {"DEDISP_FDD_M16CFG": "0",
                                                                                 "DEDISP_FDD_MIRROR": "0"}),
@@ -309,7 +311,7 @@ def test_fdd_tile_kernels_vs_oracle_chunk_counts(dd, orc, monkeypatch, prec, nch
                       ("pipe-direct", {"DEDISP_FDD_M16CFG": "2", "DEDISP_FDD_MIRROR": "0"}),
                       ("tile32", {"DEDISP_FDD_MFMA32": "1"})):
         for k in ("DEDISP_FDD_MIRROR", "DEDISP_FDD_MFMA32", "DEDISP_FDD_PACKED", "DEDISP_FDD_QTABLE",
-                  "DEDISP_FDD_M16CFG"):
+                  "DEDISP_FDD_M16CFG", "DEDISP_FDD_BLOCKED"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
