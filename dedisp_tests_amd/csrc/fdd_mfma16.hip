@@ -506,6 +506,7 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
         // Delays of the lane's 8 channels for its MT tiles: (int)(dm * delay[c])
         // in float (the conversion truncates), the reference's integer delay.
         constexpr int NU = NDIR;                // units (computed bins) per K-step
+        constexpr bool BPREF = false;           // B fragments read one product group ahead
         uint32_t d[MT][8];
         float4 dlv[2];
         auto load_dl = [&](uint32_t ks) __attribute__((always_inline)) {
@@ -546,17 +547,20 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
                 A[mt][1][j] = pack_f16x2(ci[0], ci[1]);
             }
         };
-        // the products of slot s, N-tile t with phasor planes A, staged buffer cur
-        auto group_products = [&](const char* cur, int s, int t, const APl& A) __attribute__((always_inline)) {
+        // the B fragments (staged planes) of slot s, N-tile t in buffer cur
+        using BFr = V[PL::N];
+        auto read_b = [&](const char* cur, int s, int t, BFr& x) __attribute__((always_inline)) {
+            const char* sb = cur + s * G::SSTR + b_off + t * 16 * COLB;
+#pragma unroll
+            for (int pl = 0; pl < PL::N; ++pl) x[pl] = *reinterpret_cast<const V*>(sb + pl * G::PSTR);
+        };
+        // the products of slot s, N-tile t with phasor planes A and fragments x
+        auto group_products = [&](int s, int t, const APl& A, const BFr& x) __attribute__((always_inline)) {
             V a[MT][PL::N];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int pl = 0; pl < PL::N; ++pl) a[mt][pl] = __builtin_bit_cast(V, A[mt][pl]);
-            const char* sb = cur + s * G::SSTR + b_off + t * 16 * COLB;
-            V x[PL::N];
-#pragma unroll
-            for (int pl = 0; pl < PL::N; ++pl) x[pl] = *reinterpret_cast<const V*>(sb + pl * G::PSTR);
             if constexpr (PREC == 1) {
                 // planes: 0 Ar hi, 1 Ar lo, 2 Ai hi, 3 Ai lo; x: 0 BR hi, 1 BR lo, 2 BI hi, 3 BI lo
 #pragma unroll
@@ -654,8 +658,13 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
             constexpr int SPU = MIRROR ? 2 : 1;
             constexpr int NG = SPU * NTL, NPC = MT * 4;
             const uint32_t kbn = last ? k0 : k0 + u + 1;
+            // B fragments one group ahead (BPREF), else read before use
+            BFr xb[BPREF ? 2 : 1];
+            auto gslot = [&](int gi) { return gi < NTL ? u : NDIR + u; };
+            if (BPREF) read_b(cur, gslot(0), 0, xb[0]);
 #pragma unroll
             for (int gi = 0; gi < NG; ++gi) {
+                if (BPREF && gi + 1 < NG) read_b(cur, gslot(gi + 1), (gi + 1) % NTL, xb[(gi + 1) & 1]);
 #pragma unroll
                 for (int pc = gi * NPC / NG; pc < (gi + 1) * NPC / NG; ++pc) gen_piece(kbn, Anext, pc);
                 if (MIRROR && gi == NTL) {
@@ -668,7 +677,8 @@ __global__ __launch_bounds__(WAVES * 64, 512 / (WAVES * 64)) void fdd_mfma16_k(
                             for (int pl = 0; pl < PL::N; ++pl)
                                 Acur[mt][pl][j] ^= sm[mt][j] ^ ((pl >= PL::N / 2) ? 0x80008000u : 0u);
                 }
-                group_products(cur, gi < NTL ? u : NDIR + u, gi % NTL, Acur);
+                if (!BPREF) read_b(cur, gslot(gi), gi % NTL, xb[0]);
+                group_products(gslot(gi), gi % NTL, Acur, xb[BPREF ? (gi & 1) : 0]);
             }
             // the next K-step's spectra into the other buffer (the last K-step
             // stores its own again, unread)
