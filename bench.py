@@ -2,24 +2,30 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-Workload (the headline configuration): 1024 channels from 1530 MHz down in
-0.3125 MHz steps, dt = 54.6 us, 2^20 4-bit samples per channel (Gaussian
-N(8, 2.5) quantised to 0..15, five dispersed pulses), the DM list of
-dedisp_generate_dm_list(0, 1000, 40 us, 1.25) = 1167 trials, 32-bit output.
-A step is one dedisp_execute of the TDD plan with input and output resident in
-HBM (DEDISP_DEVICE_POINTERS): the transpose kernel plus the channel-sum kernel.
-Value: DM-trials x output samples per second.
+Headline workload (the FDD precision form): 1024 channels from 1530 MHz down in
+0.3125 MHz steps, dt = 54.6 us, 2^20 8-bit samples per channel (N(128, 20)
+with five dispersed pulses), 2048 linear DM trials 0..1023.5, overlap-save
+segments of 2^16 samples, time-series output. A step is one dedisp_execute of
+the FDD plan with the bfloat16 matrix-core contraction (hi/lo split phasors
+and spectra, three products, float32 accumulation), input and output resident
+in HBM (DEDISP_DEVICE_POINTERS): transpose, forward transforms, contraction,
+inverse transforms. Value: DM-trials x output samples per second. Parity: the
+rel-L2 error against the FP64 oracle on 8 spread DM rows must stay within
+1e-3 and the S/N of the five pulses within 0.5 %; the bench exits non-zero
+otherwise.
 
 Prints one JSON line. Besides the contract fields it carries:
-  fdd            the FDD path on 1024 ch x 2^20 8-bit samples, 2048 DMs, chunks
-                 of 2^16: float32 MFMA, bfloat16 three-product MFMA and float16
-                 MFMA contraction forms, each with its error against the FP64
-                 oracle and its matrix-core roofline
-  roofline       HBM view of the channel-sum kernel (algorithmic bytes / launch
-                 time measured with HIP events on the kernel's stream)
-  valu_roofline  the integer-add view of the same kernel (channel-adds / s
-                 against the int32 VALU lane rate; the kernel packs 4 sums per
-                 32-bit lane, so the fraction can exceed 1)
+  roofline       matrix-core view of the contraction kernel: algorithmic flops
+                 (8 per complex multiply-add) / its time measured with HIP
+                 events on the plan's stream; issued_frac counts the three
+                 products and the padded chunk tiles the kernel issues
+  fdd            the float32 and float16 contraction forms on the same
+                 workload, each with its error against the FP64 oracle and its
+                 roofline, and the reference fork's default Fourier output
+  tdd            TDD on 1024 ch x 2^20 4-bit samples, the 1167 trials of
+                 dedisp_generate_dm_list(0, 1000, 40 us, 1.25): bit for bit
+                 against the oracle on 10 rows, HBM and integer-add rooflines
+                 of the channel-sum kernel, and its host-pointer run
   batch          64 beams of uneven length through one TDD graph replay, and
                  FDD-BF16 on all 64 with their chunks concatenated (one graph),
                  with its error against the FP64 oracle
@@ -29,16 +35,13 @@ Prints one JSON line. Besides the contract fields it carries:
                  FDD-BF16, FDD-FP32)
   end_to_end     the headline execute with host pointers (the reference's
                  calling convention): PCIe-inclusive rate and stage times
-  filterbank     the dedisp_fil tool on the headline input written as a
-                 SIGPROC file: read, dedisperse 64 trials, write .dat/.inf
+  filterbank     the dedisp_fil tool on the TDD input written as a SIGPROC
+                 file: read, dedisperse 64 trials, write .dat/.inf
   measured_peaks achievable HBM copy bandwidth and bf16 / fp32 GEMM rates on
                  this GPU; the rooflines also give fractions of these
   cpu_baseline   the reference's CPU FDD plan (restated as written, OpenMP)
-                 timed on this host on a bounded DM sample, scaled to the
-                 full DM list
-  parity         DM 0 checked bit for bit against an independent channel
-                 sum and 9 spread DM rows (the largest DM included) against the
-                 oracle; the bench exits non-zero on a mismatch
+                 timed on this host on a bounded DM sample of the headline
+                 DM list, scaled to all 2048 trials
 """
 from __future__ import annotations
 
@@ -160,6 +163,19 @@ def cpu_baseline(seconds_budget: float = 30.0):
 MFMA_PEAK_T = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # dense TFLOP/s
 
 
+def mfma16_tiles(nchunk: int) -> tuple:
+    """N-tiles per workgroup and chunk groups of the 16 x 16 tile contraction
+    for nchunk chunks (the kernel's rule: fewest tiles plus 1.5 tiles' worth
+    of phasor work per group, at most 3 tiles)."""
+    best, best_ntl = None, 1
+    for ntl in (1, 2, 3):
+        groups = -(-nchunk // (8 * ntl))
+        cost = groups * ntl + 1.5 * groups
+        if best is None or cost < best or (cost == best and ntl > best_ntl):
+            best, best_ntl = cost, ntl
+    return best_ntl, -(-nchunk // (8 * best_ntl))
+
+
 def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), e2e: bool = True) -> dict:
     """FDD on 1024 ch x 2^20 8-bit samples (N(128,20), five pulses), 2048 linear DM
     trials 0..1023.5, overlap-save chunks of 2^16 samples, time-series output,
@@ -190,7 +206,7 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
     del s
     dms = (np.arange(2048, dtype=np.float32) * np.float32(0.5)).astype(np.float32)
     host_in = d_in.cpu().numpy()
-    sample_rows = [0, 700, 1400, 2047]
+    sample_rows = np.linspace(0, 2047, 8).astype(int).tolist()
     otab = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
     ref = oracle.fdd_timeseries(NCH, otab, dms[sample_rows], NSAMPS, 0, host_in)
     # the DM trials nearest the injected pulses, for the S/N comparison
@@ -245,7 +261,8 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
         cmacs = prof[-1]["work_units"]
         tflops = 8 * cmacs / (c_ms * 1e-3) / 1e12
         out[name] = {
-            "value": 2048 * nsc / el, "ms_per_step": 1e3 * el,
+            "value": 2048 * nsc / el, "ms_per_step": 1e3 * el, "steps": steps,
+            "max_delay": maxd, "nchunk": nchunk, "rows_checked": sample_rows, "pulse_rows": pulse_rows,
             "stages_ms": st_ms,
             "stages_hbm": {k: {"algorithmic_bytes": v, "gbs": v / (st_ms[k] * 1e-3) / 1e9}
                            for k, v in st_bytes.items()},
@@ -254,9 +271,20 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
             "pulse_snr_note": f"max over the 5 injected pulses of |S/N / S/N(FP64) - 1| on the nearest DM trial "
                               f"(rows {pulse_rows}); S/N = (max - median) / std of the row",
             "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_PEAK_T[name], "unit": "TFLOP/s",
-                         "frac": tflops / MFMA_PEAK_T[name],
-                         "note": "algorithmic flops (8 per complex MAC, no padding or compensation products)"},
+                         "frac": tflops / MFMA_PEAK_T[name], "kernel": "contraction", "kernel_ms": c_ms,
+                         "algorithmic_flops": 8 * cmacs,
+                         "note": "algorithmic flops: 8 per complex multiply-add of DMs x channels x chunks x "
+                                 "bins 0..nfft/2 (no padding, no compensation products); the kernel time is "
+                                 "measured with HIP events on the plan's stream over the timed executes"},
         }
+        if name == "bf16":
+            # matrix-core work the kernel issues: three products per complex
+            # multiply-add (hi x hi, hi x lo, lo x hi) over chunk groups padded
+            # to 8-chunk N-tiles
+            ntl, ngroups = mfma16_tiles(nchunk)
+            issued = 8 * cmacs * 3 * (8 * ntl * ngroups) / nchunk
+            out[name]["roofline"]["issued_flops"] = issued
+            out[name]["roofline"]["issued_frac"] = issued / (c_ms * 1e-3) / 1e12 / MFMA_PEAK_T[name]
         if name == "bf16" and e2e:
             # end-to-end scope: the same execute with the caller's pageable host buffers
             h_out = np.zeros((2048, nsc), np.float32)
@@ -311,8 +339,6 @@ def fdd_leg(torch, dd, steps: int, warmup: int, forms=("fp32", "bf16", "fp16"), 
                     "value in DM x (nsamps - max_delay) per second like the other forms"}
         del d_out, plan
         torch.cuda.empty_cache()
-    if "fp32" in out and "bf16" in out:
-        out["speedup_bf16_vs_fp32"] = out["bf16"]["value"] / out["fp32"]["value"]
     if "fp32" in out and "fp16" in out:
         out["speedup_fp16_vs_fp32"] = out["fp16"]["value"] / out["fp32"]["value"]
     return out
@@ -669,19 +695,116 @@ def valu_roofline(k_ms: float, adds_t: float) -> dict:
     return out
 
 
+BF16_TOL = 1e-3        # stated tolerance of the bfloat16 form: rel-L2 vs the FP64 oracle
+SNR_TOL = 0.005        # injected-pulse S/N preserved to 0.5 %
+METRIC = "DM·samples/s: FDD BF16-MFMA vs FDD FP32 vs TDD at stated err; roofline %"
+CONTRACTION_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                   "fdd-bf16-1024ch-2p20x8bit-2048dm_contraction-counters.md")
+
+
+def contraction_traffic():
+    """HBM bytes per launch of the bfloat16 contraction kernel (FETCH_SIZE /
+    WRITE_SIZE, copy-calibrated) from the committed counter summary, or None."""
+    try:
+        with open(CONTRACTION_PROFILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def tdd_leg(torch, dd, steps: int, warmup: int) -> tuple:
+    """TDD on 1024 ch x 2^20 4-bit samples, the 1167-trial DM list of
+    dedisp_generate_dm_list(0, 1000, 40 us, 1.25), 32-bit output, input and
+    output in HBM; parity bit for bit on DM 0 (an independent channel sum) and
+    9 spread rows against the oracle. Returns (leg dict, plan, packed input,
+    output)."""
+    s, d_in = make_input(torch)
+    plan = dd.Plan(NCH, DT, F0, DF, impl=dd.TDD)
+    plan.generate_dm_list(0.0, 1000.0, 40.0, 1.25)
+    ndm, maxd = plan.dm_count, plan.max_delay
+    nsc = NSAMPS - maxd
+    d_out = torch.empty((ndm, nsc), dtype=torch.float32, device="cuda")
+    plan.set_profiling(True)
+    torch.cuda.synchronize()
+
+    def step():
+        rc = plan.execute_raw(NSAMPS, d_in, NBITS, d_out, 32, dd.DEVICE_POINTERS)
+        if rc != 0:
+            raise dd.DedispError(rc, "dedisp_execute")
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    kernel_ms, unpack_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        p = plan.profile()
+        kernel_ms.append(p["dedisperse_ms"] / max(1, p["dedisperse_launches"]))
+        unpack_ms.append(p["unpack_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    # parity: DM 0 is the plain channel sum (independent torch sum), and 9
+    # spread DM rows, the largest DM included, equal the oracle bit for bit
+    row0 = d_out[0].cpu().numpy()
+    sums = s[:nsc].to(torch.int32).sum(dim=1).cpu().numpy().astype(np.float32)
+    f32 = np.float32
+    factor = (f32(3.0) * f32(1024.0)) / f32(255.0) / f32(16.0)
+    ref0 = ((sums * f32(1.0)) / (f32(15.0) * f32(NCH)) * factor).astype(f32)
+    import oracle
+    parity_rows = sorted(set(np.linspace(0, ndm - 1, 9).astype(int).tolist()))
+    host_packed = d_in.cpu().numpy()
+    otab = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
+    ref_rows = oracle.tdd(NCH, otab, None, plan.dm_list()[parity_rows], NSAMPS, maxd, host_packed, NBITS, 32)
+    got_rows = d_out[parity_rows].cpu().numpy()
+    bad = [r for j, r in enumerate(parity_rows) if not np.array_equal(got_rows[j], ref_rows[j])]
+    parity = "ok" if (np.array_equal(row0, ref0) and not bad) else "MISMATCH"
+    del host_packed, got_rows, ref_rows, s
+
+    k_ms = float(np.mean(kernel_ms))
+    in_bytes = NSAMPS * NCH * NBITS / 8
+    out_bytes = ndm * nsc * 4
+    achieved_gbs = (in_bytes + out_bytes) / (k_ms * 1e-3) / 1e9
+    adds_t = ndm * nsc * NCH / (k_ms * 1e-3) / 1e12
+    leg = {
+        "value": ndm * nsc / (elapsed / steps), "unit": "DM*samples/s", "ms_per_step": 1e3 * elapsed / steps,
+        "steps": steps, "dtype": "u32",
+        "config": {"workload": "tdd-1024ch-2^20x4bit-1167dm", "nchans": NCH, "nsamps": NSAMPS,
+                   "in_nbits": NBITS, "out_nbits": 32, "ndm": ndm, "max_delay": maxd,
+                   "dm_list": "dedisp_generate_dm_list(0,1000,40us,1.25)"},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": "profiles/tdd-1024ch-2p20x4bit-1167dm_hbm-traffic.md",
+                     "kernel": "tdd_lds_k", "kernel_ms": k_ms, "algorithmic_bytes": in_bytes + out_bytes,
+                     "note": "integer-add work bounds this kernel, not HBM: see valu_roofline"},
+        "valu_roofline": valu_roofline(k_ms, adds_t),
+        "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
+        "parity": parity,
+        "parity_detail": {"rows_checked": len(parity_rows) + 1, "rows": [0] + parity_rows, "mismatched_rows": bad,
+                          "against": "oracle TDD (bit for bit) and an independent channel sum for DM 0"},
+    }
+    return leg, plan, d_in, d_out, ndm, nsc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fdd", action="store_true")
+    ap.add_argument("--no-tdd", action="store_true")
+    ap.add_argument("--no-fdd", action="store_true",
+                    help="run the TDD leg only and print it (profiling the TDD kernels)")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip the host-pointer runs (profiling: they launch the same kernels on DM blocks)")
     ap.add_argument("--fdd-forms", default="fp32,bf16,fp16",
-                    help="contraction forms of the FDD leg (counter runs use one)")
+                    help="contraction forms of the FDD leg besides the headline bfloat16 (counter runs use bf16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device to device twice (the known-size kernel that "
                          "scales FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -704,121 +827,94 @@ def main():
         return
 
     t_start = time.perf_counter()
-    s, d_in = make_input(torch)
-    plan = dd.Plan(NCH, DT, F0, DF, impl=dd.TDD)
-    plan.generate_dm_list(0.0, 1000.0, 40.0, 1.25)
-    ndm, maxd = plan.dm_count, plan.max_delay
-    nsc = NSAMPS - maxd
-    d_out = torch.empty((ndm, nsc), dtype=torch.float32, device="cuda")
-    plan.set_profiling(True)
-    torch.cuda.synchronize()
-
-    def step():
-        rc = plan.execute_raw(NSAMPS, d_in, NBITS, d_out, 32, dd.DEVICE_POINTERS)
-        if rc != 0:
-            raise dd.DedispError(rc, "dedisp_execute")
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    kernel_ms, unpack_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        p = plan.profile()
-        kernel_ms.append(p["dedisperse_ms"] / max(1, p["dedisperse_launches"]))
-        unpack_ms.append(p["unpack_ms"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = ndm * nsc / (elapsed / args.steps)
-
-    # parity: DM 0 is the plain channel sum (independent torch sum), and 9
-    # spread DM rows, the largest DM included, equal the oracle bit for bit
-    row0 = d_out[0].cpu().numpy()
-    sums = s[:nsc].to(torch.int32).sum(dim=1).cpu().numpy().astype(np.float32)
-    f32 = np.float32
-    factor = (f32(3.0) * f32(1024.0)) / f32(255.0) / f32(16.0)
-    ref0 = ((sums * f32(1.0)) / (f32(15.0) * f32(NCH)) * factor).astype(f32)
-    import oracle
-    parity_rows = sorted(set(np.linspace(0, ndm - 1, 9).astype(int).tolist()))
-    host_packed = d_in.cpu().numpy()
-    otab = oracle.delay_table(NCH, np.float32(DT), np.float32(F0), np.float32(DF))
-    ref_rows = oracle.tdd(NCH, otab, None, plan.dm_list()[parity_rows], NSAMPS, maxd, host_packed, NBITS, 32)
-    got_rows = d_out[parity_rows].cpu().numpy()
-    bad = [r for j, r in enumerate(parity_rows) if not np.array_equal(got_rows[j], ref_rows[j])]
-    parity = "ok" if (np.array_equal(row0, ref0) and not bad) else "MISMATCH"
-    parity_detail = {"rows_checked": len(parity_rows) + 1, "rows": [0] + parity_rows, "mismatched_rows": bad,
-                     "against": "oracle TDD (bit for bit) and an independent channel sum for DM 0"}
-    del host_packed, got_rows, ref_rows
-
-    k_ms = float(np.mean(kernel_ms))
-    in_bytes = NSAMPS * NCH * NBITS / 8
-    out_bytes = ndm * nsc * 4
-    achieved_gbs = (in_bytes + out_bytes) / (k_ms * 1e-3) / 1e9
-    adds = ndm * nsc * NCH
-    adds_t = adds / (k_ms * 1e-3) / 1e12
-
+    if args.no_fdd:
+        tdd = tdd_leg(torch, dd, args.steps, args.warmup)[0]
+        print(json.dumps({"tdd": tdd}))
+        sys.exit(0 if tdd["parity"] == "ok" else 1)
+    # ---- headline: FDD with the bfloat16 matrix-core contraction ----
+    forms = tuple(dict.fromkeys(["bf16"] + [f for f in args.fdd_forms.split(",") if f]))
+    fdd = fdd_leg(torch, dd, args.steps, args.warmup, forms=forms, e2e=not args.no_e2e)
+    b = fdd.pop("bf16")
+    parity = "ok" if (b["rel_l2_vs_fp64"] <= BF16_TOL and b["pulse_snr_rel_err"] <= SNR_TOL) else "MISMATCH"
+    roof = dict(b["roofline"])
+    roof["traffic"] = contraction_traffic()
+    roof["traffic_source"] = os.path.relpath(CONTRACTION_PROFILE, os.path.dirname(os.path.abspath(__file__)))
     line = {
-        "metric": "DM*samples/s (TDD, bit-exact)",
-        "value": value,
+        "metric": METRIC,
+        "value": b["value"],
         "unit": "DM*samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": "tdd-1024ch-2^20x4bit-1167dm", "nchans": NCH, "nsamps": NSAMPS,
-                   "in_nbits": NBITS, "out_nbits": 32, "ndm": ndm, "max_delay": maxd,
-                   "dm_list": "dedisp_generate_dm_list(0,1000,40us,1.25)"},
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "traffic_source": "profiles/tdd-1024ch-2p20x4bit-1167dm_hbm-traffic.md",
-                     "kernel": "tdd_lds_k", "kernel_ms": k_ms,
-                     "algorithmic_bytes": in_bytes + out_bytes,
-                     "note": "integer-add work bounds this kernel, not HBM: see valu_roofline"},
-        "valu_roofline": valu_roofline(k_ms, adds_t),
-        "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
+        "config": {"workload": fdd["workload"], "nchans": NCH, "nsamps": NSAMPS, "in_nbits": 8,
+                   "out_nbits": 32, "ndm": 2048, "dm_list": "linear 0..1023.5 step 0.5", "nfft": 65536,
+                   "max_delay": b["max_delay"], "nchunk": b["nchunk"], "output": "time series",
+                   "contraction": "bfloat16 MFMA, hi/lo split phasors and spectra (three products), "
+                                  "float32 accumulation"},
+        "roofline": roof,
+        "stages_ms": b["stages_ms"],
+        "stages_hbm": b["stages_hbm"],
+        "rel_l2_vs_fp64": b["rel_l2_vs_fp64"],
+        "max_abs_vs_fp64": b["max_abs_vs_fp64"],
+        "pulse_snr_rel_err": b["pulse_snr_rel_err"],
+        "tolerance": {"rel_l2": BF16_TOL, "pulse_snr": SNR_TOL},
         "parity": parity,
-        "parity_detail": parity_detail,
+        "parity_detail": {"rows": b["rows_checked"], "pulse_rows": b["pulse_rows"],
+                          "against": "oracle.fdd_timeseries (FP64 delays, phases and transforms)"},
     }
-    progress("tdd", t_start)
+    if "end_to_end" in b:
+        line["end_to_end"] = b["end_to_end"]
+    line["fdd"] = fdd
+    if "fp32" in fdd:
+        line["fdd"]["speedup_bf16_vs_fp32"] = b["value"] / fdd["fp32"]["value"]
+    progress("fdd", t_start)
     if parity != "ok":
         print(json.dumps(line))
-        print(f"bench: TDD output differs from the oracle on rows {bad}", file=sys.stderr)
+        print(f"bench: FDD-BF16 error {b['rel_l2_vs_fp64']:.3g} / pulse S/N {b['pulse_snr_rel_err']:.3g} "
+              "outside the stated tolerance", file=sys.stderr)
         sys.exit(1)
+    torch.cuda.empty_cache()
     peaks = peaks_leg(torch)
     line["measured_peaks"] = peaks
-    line["roofline"]["frac_of_measured_peak"] = achieved_gbs / peaks["hbm_copy_gbs"]
-    vr = line["valu_roofline"]
-    vr["adds_frac_of_measured_int_add"] = vr["adds_per_s"] / peaks["int_add_lane_adds_per_s"]
-    if vr["achieved"] is not None:
-        vr["frac_of_measured_issue"] = vr["achieved"] / peaks["valu_wave_insts_per_s"]
+    line["roofline"]["frac_of_measured_gemm"] = line["roofline"]["achieved"] / peaks["bf16_gemm_tflops"]
+    for form, key in (("fp32", "fp32_gemm_tflops"), ("fp16", "fp16_gemm_tflops")):
+        if form in line["fdd"]:
+            rl = line["fdd"][form]["roofline"]
+            rl["frac_of_measured_gemm"] = rl["achieved"] / peaks[key]
+    for st in line["stages_hbm"].values():
+        st["frac_of_measured_hbm"] = st["gbs"] / peaks["hbm_copy_gbs"]
     progress("peaks", t_start)
-    if not args.no_e2e:
-        host_in = d_in.cpu().numpy()
-        line["end_to_end"] = e2e_leg(dd, plan, host_in, d_out, ndm, nsc)
-        progress("end_to_end", t_start)
-        line["filterbank"] = fil_leg(host_in)
-        progress("filterbank", t_start)
-        del host_in
-    del d_out, d_in, s
-    torch.cuda.empty_cache()
-    if not args.no_fdd:
-        line["fdd"] = fdd_leg(torch, dd, max(2, args.steps // 3), 1 if args.warmup else 0,
-                              forms=args.fdd_forms.split(","), e2e=not args.no_e2e)
-        for form, key in (("fp32", "fp32_gemm_tflops"), ("bf16", "bf16_gemm_tflops"), ("fp16", "fp16_gemm_tflops")):
-            if form in line["fdd"]:
-                rl = line["fdd"][form]["roofline"]
-                rl["frac_of_measured_gemm"] = rl["achieved"] / peaks[key]
-                for st in line["fdd"][form]["stages_hbm"].values():
-                    st["frac_of_measured_hbm"] = st["gbs"] / peaks["hbm_copy_gbs"]
+
+    # ---- TDD beside it (bit-exact) ----
+    if not args.no_tdd:
+        tdd, plan, d_in, d_out, ndm, nsc = tdd_leg(torch, dd, args.steps, args.warmup)
+        tdd["roofline"]["frac_of_measured_peak"] = tdd["roofline"]["achieved"] / peaks["hbm_copy_gbs"]
+        vr = tdd["valu_roofline"]
+        vr["adds_frac_of_measured_int_add"] = vr["adds_per_s"] / peaks["int_add_lane_adds_per_s"]
+        if vr["achieved"] is not None:
+            vr["frac_of_measured_issue"] = vr["achieved"] / peaks["valu_wave_insts_per_s"]
+        line["tdd"] = tdd
+        progress("tdd", t_start)
+        if tdd["parity"] != "ok":
+            print(json.dumps(line))
+            print(f"bench: TDD output differs from the oracle on rows {tdd['parity_detail']['mismatched_rows']}",
+                  file=sys.stderr)
+            sys.exit(1)
+        if not args.no_e2e:
+            host_in = d_in.cpu().numpy()
+            tdd["end_to_end"] = e2e_leg(dd, plan, host_in, d_out, ndm, nsc)
+            progress("end_to_end", t_start)
+            line["filterbank"] = fil_leg(host_in)
+            progress("filterbank", t_start)
+            del host_in
+        del d_out, d_in, plan
         torch.cuda.empty_cache()
-        progress("fdd", t_start)
     if not args.no_batch:
         line["batch"] = batch_leg(torch, dd)
         torch.cuda.empty_cache()
