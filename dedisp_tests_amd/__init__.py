@@ -81,7 +81,8 @@ class Tuning(C.Structure):
     """dedisp_amd_tuning (include/dedisp_amd.h)."""
     _fields_ = [("samples_per_lane", C.c_uint), ("dms_per_wave", C.c_uint), ("waves", C.c_uint),
                 ("chunk_channels", C.c_uint), ("nfft", C.c_ulong), ("best_ms", C.c_double),
-                ("candidates", C.c_uint)]
+                ("candidates", C.c_uint), ("fdd_form", C.c_int), ("fdd_mirror", C.c_int),
+                ("fdd_layout", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

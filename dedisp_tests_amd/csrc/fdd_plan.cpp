@@ -85,6 +85,11 @@ struct FDDState {
     DeviceBuffer d_twN;     // [N/2] e^{-2 pi i k/N}
     size_t tw_N = 0;
 
+    // contraction choices of the autotuner for precision tuned_prec (-1:
+    // none): form 0/1/2 = the 16 x 16 tile kernel's workgroup shape, 3 = the
+    // 32 x 32 tile kernel; mirror bins; bin-block layout (0 = rows)
+    int tuned_prec = -1, tuned_form = -1, tuned_mirror = -1, tuned_blocked = -1;
+
     unsigned long version = ~0ul;
     int32_t dmin = 0, dmax = 0;
     size_t ph_nfft = 0;
@@ -112,20 +117,41 @@ bool env_flag(const char* name)
 // 32 x 32 tile kernel instead), float32 on 32 x 32 x 2 tiles
 // (DEDISP_FDD_FP32_TILE16=1: 16 x 16 x 4 tiles with re/im along N), and
 // DEDISP_FDD_VALU=1 the float32 vector-ALU form.
+// The autotuner's choices (FDDGPUPlan::autotune) replace these defaults for
+// the precision they were timed with; the environment variables override both.
 enum class Contraction { VALU, MFMA32, MFMA16 };
 
-Contraction contraction_kind(dedisp_amd_fdd_precision p)
+// Whether the autotuner's choices apply (timed for the plan's precision).
+bool tuned(const FDDState& st) { return st.tuned_prec == (int)st.precision; }
+
+Contraction contraction_kind(const FDDState& st)
 {
-    if (p == DEDISP_AMD_FDD_FP32) {
+    const bool t = tuned(st) && st.tuned_form >= 0;
+    if (st.precision == DEDISP_AMD_FDD_FP32) {
         if (env_flag("DEDISP_FDD_VALU")) return Contraction::VALU;
-        return env_flag("DEDISP_FDD_FP32_TILE16") ? Contraction::MFMA16 : Contraction::MFMA32;
+        if (env_flag("DEDISP_FDD_FP32_TILE16")) return Contraction::MFMA16;
+        return (t && st.tuned_form != 3) ? Contraction::MFMA16 : Contraction::MFMA32;
     }
-    return env_flag("DEDISP_FDD_MFMA32") ? Contraction::MFMA32 : Contraction::MFMA16;
+    if (env_flag("DEDISP_FDD_MFMA32")) return Contraction::MFMA32;
+    return (t && st.tuned_form == 3) ? Contraction::MFMA32 : Contraction::MFMA16;
 }
 
 // Channel rows of the spectra: a multiple of 32 (the K-step of the 16 x 16
 // tile kernel, a multiple of the 32 x 32 kernel's 16), extra rows zero.
 size_t padded_channels(size_t nchan) { return round_up(nchan, 32); }
+
+// DMs per batch of the contraction and inverse when at most `most` fit the
+// memory budget: as many batches as needed, of equal size rounded up to the
+// contraction's 256-DM blocks (2048 DMs with room for 1730: 2 x 1024, not
+// 1730 + 318, whose second batch leaves most of its DM blocks idle).
+size_t balanced_batch(size_t ndm, size_t most)
+{
+    most = std::max<size_t>(1, std::min(ndm, most));
+    const size_t nbat = div_up(ndm, most);
+    const size_t even = div_up(ndm, nbat);
+    const size_t blk = round_up(even, 256);
+    return (blk <= most) ? blk : even;
+}
 
 }  // namespace
 
@@ -218,19 +244,70 @@ double FDDGPUPlan::autotune(size_type nsamps, size_type in_nbits, size_type out_
         (void)hipEventDestroy(e1);
         throw;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (best >= 0) {
         set_segmented(1, best_n);
     } else {
         m_state->segmented = saved_seg;
         m_state->nfft_req = saved_nfft;
     }
+    // the contraction at that segment length: workgroup form (0, 1, 2 = the
+    // 16 x 16 tile kernel's 8-wave, 4-wave and pipelined shapes, 3 = the
+    // 32 x 32 tile kernel), mirror bins, bin-block layout (-1: not used)
+    FDDState& st = *m_state;
+    struct Cand {
+        int form, mirror, blocked;
+    };
+    std::vector<Cand> cands;
+    if (st.precision == DEDISP_AMD_FDD_FP32)
+        cands = {{3, -1, -1}, {0, 0, -1}, {0, 1, -1}};
+    else
+        cands = {{2, 1, 1}, {2, 1, 0}, {2, 0, -1}, {0, 1, -1}, {0, 0, -1}, {3, -1, -1}};
+    const int saved[4] = {st.tuned_prec, st.tuned_form, st.tuned_mirror, st.tuned_blocked};
+    Cand pick = {-1, -1, -1};
+    double pick_ms = -1;
+    try {
+        if (best >= 0) {
+            for (const Cand& c : cands) {
+                st.tuned_prec = (int)st.precision;
+                st.tuned_form = c.form;
+                st.tuned_mirror = c.mirror;
+                st.tuned_blocked = c.blocked;
+                execute(nsamps, d_in.as<uint8_t>(), in_nbits, d_out.as<uint8_t>(), 32, DEDISP_DEVICE_POINTERS);
+                DD_HIP(hipEventRecord(e0, ctx.exec));
+                execute(nsamps, d_in.as<uint8_t>(), in_nbits, d_out.as<uint8_t>(), 32, DEDISP_DEVICE_POINTERS);
+                DD_HIP(hipEventRecord(e1, ctx.exec));
+                DD_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ++timed;
+                if (pick_ms < 0 || ms < pick_ms) pick_ms = ms, pick = c;
+            }
+        }
+    } catch (...) {
+        st.tuned_prec = saved[0], st.tuned_form = saved[1], st.tuned_mirror = saved[2], st.tuned_blocked = saved[3];
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (pick_ms >= 0) {
+        st.tuned_prec = (int)st.precision;
+        st.tuned_form = pick.form;
+        st.tuned_mirror = pick.mirror;
+        st.tuned_blocked = pick.blocked;
+        best = pick_ms;
+    } else {
+        st.tuned_prec = saved[0], st.tuned_form = saved[1], st.tuned_mirror = saved[2], st.tuned_blocked = saved[3];
+    }
     if (chosen) {
         *chosen = dedisp_amd_tuning{};
         chosen->nfft = best_n;
         chosen->best_ms = best;
         chosen->candidates = timed;
+        chosen->fdd_form = pick.form;
+        chosen->fdd_mirror = pick.mirror;
+        chosen->fdd_layout = pick.blocked;
     }
     return best;
 }
@@ -299,7 +376,7 @@ size_t FDDGPUPlan::segment_length(const size_t* nsc, size_t n) const
     if (!nfft) {
         // smallest contraction volume (chunk columns the kernel computes,
         // padding included) x (nfft/2 + 1) among the supported lengths
-        const Contraction kind = contraction_kind(st.precision);
+        const Contraction kind = contraction_kind(st);
         size_t best = ~size_t(0);
         for (size_t cand = 8192; cand <= 262144; cand *= 2) {
             if (cand < dmax + 2 + 2) continue;
@@ -407,12 +484,14 @@ int m16cfg_env()
 // delays computed in the kernel while a K-step's spectra stay within 32-bit
 // element offsets; else 0 (8 waves of 2 DM tiles, 4 slots). DEDISP_FDD_M16CFG
 // forces 0, 1 (4 waves of 4 DM tiles, 2 slots) or 2 where it applies.
-int mfma16_cfg(dedisp_amd_fdd_precision prec, size_t nchunk, size_t Xp)
+int mfma16_cfg(const FDDState& st, size_t nchunk, size_t Xp)
 {
-    const bool pipe = prec != DEDISP_AMD_FDD_FP32 && !env_flag("DEDISP_FDD_QTABLE") &&
+    const bool pipe = st.precision != DEDISP_AMD_FDD_FP32 && !env_flag("DEDISP_FDD_QTABLE") &&
                       32 * nchunk * Xp + Xp < (size_t(1) << 32);
     const int e = m16cfg_env();
     if (e == 0 || e == 1 || (e == 2 && pipe)) return e;
+    if (tuned(st) && (st.tuned_form == 0 || st.tuned_form == 1 || (st.tuned_form == 2 && pipe)))
+        return st.tuned_form;
     return pipe ? 2 : 0;
 }
 
@@ -422,16 +501,17 @@ int mfma16_cfg(dedisp_amd_fdd_precision prec, size_t nchunk, size_t Xp)
 // the bins and which then run in the bin-block layout (1024 ch x 2048 DMs x
 // 23 chunks of 65536: float16 21.2 ms against 28.7 ms on four consecutive
 // bins); not for float32 (117.1 ms on consecutive bins against 119.5 ms).
-bool mfma16_mirror(dedisp_amd_fdd_precision prec)
+bool mfma16_mirror(const FDDState& st)
 {
     const char* s = std::getenv("DEDISP_FDD_MIRROR");
     if (s) return std::atoi(s) != 0;
-    return prec != DEDISP_AMD_FDD_FP32;
+    if (tuned(st) && st.tuned_mirror >= 0) return st.tuned_mirror != 0;
+    return st.precision != DEDISP_AMD_FDD_FP32;
 }
 
 bool contraction_packs(const FDDState& st)
 {
-    return contraction_kind(st.precision) == Contraction::MFMA16 && mfma16_mirror(st.precision) &&
+    return contraction_kind(st) == Contraction::MFMA16 && mfma16_mirror(st) &&
            !env_off("DEDISP_FDD_PACKED");
 }
 
@@ -441,8 +521,8 @@ bool contraction_packs(const FDDState& st)
 // writes whole cache lines (DEDISP_FDD_BLOCKED=0 keeps rows of Xp / Yp).
 bool blocked_layout(const FDDState& st, size_t nfft, size_t nchunk, size_t Xp)
 {
-    return contraction_packs(st) && mfma16_cfg(st.precision, nchunk, Xp) == 2 && nfft <= 65536 &&
-           !env_off("DEDISP_FDD_BLOCKED");
+    return contraction_packs(st) && mfma16_cfg(st, nchunk, Xp) == 2 && nfft <= 65536 &&
+           !env_off("DEDISP_FDD_BLOCKED") && !(tuned(st) && st.tuned_blocked == 0);
 }
 
 // Elements of one channel's spectra: rows of Xp per chunk, or the bin blocks.
@@ -460,7 +540,7 @@ int inverse_form(const FDDState& st, bool blocked) { return blocked ? 2 : (contr
 void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, size_t nfft, size_t Xp, size_t Yp,
               float xscale, bool blocked, hipStream_t stream)
 {
-    switch (contraction_kind(st.precision)) {
+    switch (contraction_kind(st)) {
     case Contraction::VALU:
         DD_HIP(dd_fdd_contract_fp32(st.d_Z.as<float2>(), Xp, st.d_delays.as<int32_t>() + b0 * nchan, nb, nchan,
                                     nchunk, st.d_ph.as<float2>(), nfft, st.d_Y.as<float2>(), Yp, stream));
@@ -475,13 +555,13 @@ void contract(FDDState& st, size_t nchan, size_t b0, size_t nb, size_t nchunk, s
         // two bins and their mirrors
         // delays from the DM list and delay table in the kernel
         // (DEDISP_FDD_QTABLE=1: read from the phase-step table instead)
-        DD_HIP(dd_fdd_contract_mfma16((int)st.precision, mfma16_mirror(st.precision), st.d_Z.as<float2>(), Xp,
+        DD_HIP(dd_fdd_contract_mfma16((int)st.precision, mfma16_mirror(st), st.d_Z.as<float2>(), Xp,
                                       st.d_q32.as<uint32_t>() + b0 * st.qpitch32, st.qpitch32, nb, nchan, nchunk,
                                       nfft, xscale, st.d_Y.as<float2>(), Yp,
                                       contraction_packs(st) ? st.d_ph.as<float2>() : nullptr,
                                       env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_dm32.as<float>() + b0,
                                       env_flag("DEDISP_FDD_QTABLE") ? nullptr : st.d_delay32.as<float>(),
-                                      mfma16_cfg(st.precision, nchunk, Xp), blocked ? 1 : 0, stream));
+                                      mfma16_cfg(st, nchunk, Xp), blocked ? 1 : 0, stream));
         break;
     }
 }
@@ -609,7 +689,7 @@ void FDDGPUPlan::execute_fdd(size_type nsamps, const byte_type* in, size_type in
 
     // ---- per DM batch: contraction, inverse transforms ----
     const size_t per_dm = nchunk * Yp * 8 + (fourier ? 2 * Nf * 4 : 0) + dd_fdd_split_work(nfft, nchunk);
-    size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    size_t dm_batch = balanced_batch(ndm, kYBudget / per_dm);
     if (!device_ptrs) {
         // host outputs: DM batches of about 1 GB of output (whole 256-DM tiles
         // of the contraction), so the download overlaps the later batches
@@ -781,7 +861,7 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
         g_chunks = std::max(g_chunks, g.nchunk);
     }
     const size_t per_dm = g_chunks * Yp * 8 + dd_fdd_split_work(nfft, g_chunks);
-    const size_t dm_batch = std::max<size_t>(1, std::min(ndm, kYBudget / per_dm));
+    const size_t dm_batch = balanced_batch(ndm, kYBudget / per_dm);
     st.d_rows.reserve(g_rows * 4);
     st.d_Z.reserve(nchan_pad * g_chunks * Xp * 8);
     st.d_Y.reserve(dm_batch * g_chunks * Yp * 8);
@@ -799,8 +879,10 @@ void FDDGPUPlan::execute_batch(size_type nbeams, const size_type* nsamps, const 
                                  (uint64_t)st.d_rows.ptr, (uint64_t)st.d_Z.ptr, (uint64_t)st.d_Y.ptr,
                                  (uint64_t)st.d_q32.ptr, (uint64_t)st.d_ph.ptr, (uint64_t)st.d_ph64k.ptr,
                                  (uint64_t)st.d_fwork.ptr, (uint64_t)st.d_iwork.ptr,
-                                 (uint64_t)contraction_kind(st.precision), (uint64_t)contraction_packs(st),
-                                 (uint64_t)mfma16_mirror(st.precision), (uint64_t)(m16cfg_env() + 1),
+                                 (uint64_t)contraction_kind(st), (uint64_t)contraction_packs(st),
+                                 (uint64_t)mfma16_mirror(st), (uint64_t)(m16cfg_env() + 1),
+                                 (uint64_t)(st.tuned_prec + 1), (uint64_t)(st.tuned_form + 1),
+                                 (uint64_t)(st.tuned_mirror + 1), (uint64_t)(st.tuned_blocked + 1),
                                  (uint64_t)env_flag("DEDISP_FDD_QTABLE"), (uint64_t)env_off("DEDISP_FDD_BLOCKED"),
                                  (uint64_t)st.d_scale.ptr, (uint64_t)st.d_delays.ptr};
     for (size_t b = 0; b < nbeams; ++b) {

@@ -1006,8 +1006,11 @@ __global__ __launch_bounds__(256) void fourier_step_b_k(const float2* __restrict
     if (tw_lds && m % 2 == 0) {
         // even m: one radix-2 split, E[k] / O[k] = the h-point DFTs (h = m/2)
         // of the even / odd n1, X[k] = E[k] + W_m^k O[k], X[k+h] = E[k] - W_m^k O[k];
-        // W_h^e = W_m^{2e}. KH outputs k per pass over the inputs.
-        constexpr int KH = 8;
+        // W_h^e = W_m^{2e}. KH outputs k per pass over the inputs: the pass
+        // count sets how often the column is re-read from HBM (the kernel is
+        // bound by those reads: 17 instead of 8 outputs per pass takes the
+        // 66-point step of the 2048-DM workload from 5 passes to 2)
+        constexpr int KH = 17;
         const uint32_t h = (uint32_t)(m / 2);
         for (uint32_t kb = 0; kb < h; kb += KH) {
             float2 E[KH], O[KH];

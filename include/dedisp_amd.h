@@ -213,14 +213,22 @@ typedef struct {
     dedisp_size nfft;
     double      best_ms;      /* execute time of the chosen configuration */
     unsigned    candidates;   /* configurations timed */
+    /* FDD contraction at that segment length: fdd_form 0, 1, 2 = the 16 x 16
+     * tile kernel with 8 waves, 4 waves, software-pipelined 8 waves; 3 = the
+     * 32 x 32 tile kernel. fdd_mirror: bins paired with their mirrors (1) or
+     * consecutive (0); fdd_layout: spectra in bin blocks (1) or rows (0);
+     * -1 where the form has no such choice. 0 for TDD plans. */
+    int         fdd_form, fdd_mirror, fdd_layout;
 } dedisp_amd_tuning;
 
 /* Times the plan's configurations on nsamps samples of in_nbits input in
  * device scratch (contents irrelevant to the timing) and keeps the fastest for
  * later executes: TDD / legacy plans every instantiated channel-sum tiling for
  * that width, FDD plans the overlap-save segment lengths 2^13 .. 2^18 longer
- * than max_delay (time-series output). Requires the DM list; returns the
- * choice in *result. */
+ * than max_delay (time-series output) and then, at the chosen length, the
+ * contraction forms of the plan's precision (workgroup shape, mirror bins,
+ * spectra layout; kept for that precision, the environment variables of the
+ * forms still override). Requires the DM list; returns the choice in *result. */
 dedisp_error dedisp_amd_autotune(dedisp_plan plan, dedisp_size nsamps, dedisp_size in_nbits,
                                  dedisp_size out_nbits, dedisp_amd_tuning* result);
 

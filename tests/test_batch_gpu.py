@@ -232,6 +232,37 @@ def test_autotune_fdd_segment_length(dd, orc):
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-4
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_autotune_fdd_contraction_form(dd, orc, monkeypatch, prec):
+    """The autotuner also times the contraction forms of the plan's precision
+    (workgroup shape, mirror bins, spectra layout) at the chosen segment
+    length; the kept form is one of them and the executes after it stay
+    within the precision's tolerance of the oracle."""
+    for k in ("DEDISP_FDD_MIRROR", "DEDISP_FDD_MFMA32", "DEDISP_FDD_PACKED", "DEDISP_FDD_QTABLE",
+              "DEDISP_FDD_M16CFG", "DEDISP_FDD_BLOCKED", "DEDISP_FDD_FP32_TILE16", "DEDISP_FDD_VALU"):
+        monkeypatch.delenv(k, raising=False)
+    nch, dt, f0, df = 96, 64e-6, 1581.8, -2.0
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_TIMESERIES)
+    plan.fdd_set_precision({"fp32": dd.FDD_FP32, "bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec])
+    dms = np.linspace(0, 60, 40, dtype=f32)
+    plan.set_dm_list(dms)
+    nsamps = 120000
+    r = plan.autotune(nsamps, 8, 32)
+    forms = {(3, -1, -1), (0, 0, -1), (0, 1, -1)} if prec == "fp32" else \
+        {(2, 1, 1), (2, 1, 0), (2, 0, -1), (0, 1, -1), (0, 0, -1), (3, -1, -1)}
+    assert (r["fdd_form"], r["fdd_mirror"], r["fdd_layout"]) in forms, r
+    assert r["candidates"] >= len(forms) + 1 and r["best_ms"] > 0
+    rng = np.random.default_rng(9)
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    out = plan.execute(data, 8, 32)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    tol = 1e-3 if prec == "fp16" else 1e-4
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol
+    plan.close()
+
+
 def test_batch_uneven_beams_512ch_2bit(dd, orc):
     """The batched-beams geometry (512 ch 1500 -> 1100 MHz, 81.92 us, 2-bit,
     1024 DMs 0..2000, max_delay 38585) on 6 beams of log-uniform length in
