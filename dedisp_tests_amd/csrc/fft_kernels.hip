@@ -938,7 +938,13 @@ hipError_t launch_inverse_split(const float2* Y, size_t nrows, size_t nchunk, si
 //   columns: z viewed as [n2][n1] transposed to [n1][n2] (tiles through LDS),
 //            so each column n1 is contiguous
 //   step A: 8192-point DFT over n2 for each (row, n1), times W_L^{n1 k2}
-//   step B: m-point DFT over n1 for each (row, k2)
+//   step B: m-point DFT over n1 for each (row, k2): a direct sum, O(m^2) per
+//           column (one radix-2 split for even m halves it); with the
+//           twiddles in LDS for m <= 4096 (N <= 2^26), read from twL above.
+//           At the reference workloads m is small (66 for 2^20 samples), where
+//           the step is bound by re-reading its column, not by the sums; long
+//           series grow quadratically in this step (a mixed-radix m-point
+//           transform would make it O(m log m))
 //   step C: real-input pairing X[k] = (Z[k] + conj Z[L-k])/2
 //                                     - i/2 W_N^k (Z[k] - conj Z[L-k])
 // twL[j] = e^{-2 pi i j/L}, twN[k] = e^{-2 pi i k/N} (j, k < L), built in double.
