@@ -53,13 +53,38 @@ typedef struct {
 // of each channel relative to the time tile's first word.
 hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_words, unsigned nbits,
                                  const uint32_t* delays, const int32_t* chan,
-                                 const uint32_t* wstart, size_t nact, size_t ndm,
+                                 const uint32_t* wstart, const uint32_t* chunks, unsigned maxc,
+                                 size_t nact, size_t ndm,
                                  size_t nsamps_out, uint8_t* out, size_t out_stride,
                                  dd_tdd_epilogue ep, dd_tdd_tiling tiling,
                                  hipStream_t stream);
 
 // Whether dd_tdd_dedisperse_lds has an instantiation for this tiling.
 int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tiling);
+
+// Runs-layout channel sum (tdd_runs.hip): 4-bit input, the tiling with
+// samples_per_lane 28 (7 time runs x 4 positions per lane), 4 DMs per wave
+// and 4 waves or 2 DMs per wave and 8 waves. Each DM block (16 DMs) stages its
+// channels in chunks listed in
+// chunks[block * maxc ...] (entry: first channel | channels << 16 |
+// (positions / 128) << 24; an entry with no channels ends the list); wstart:
+// [ndmblk][nact] first staged word of each channel relative to the tile's
+// first word (a multiple of 2); code: [DM group][nact][DMs per wave] LDS byte offsets of
+// each DM's window in the run words of its chunk. A chunk's staged windows fit
+// in DD_TDD_RUNS_L1_BYTES (time-ordered words, 72 LDS words per 64) and its
+// run words in DD_TDD_RUNS_L2_BYTES, with at most DD_TDD_RUNS_MAXCC channels.
+// At most 4369 active channels (16-bit sums); chan and wstart are read up to 8
+// entries past a chunk's last channel. beams / tile_beam / batch_tiles:
+// as dd_tdd_dedisperse_lds_batch (or null / 0 for one series).
+#define DD_TDD_RUNS_L1_BYTES 18432
+#define DD_TDD_RUNS_L2_BYTES 30720
+#define DD_TDD_RUNS_SPL 28
+#define DD_TDD_RUNS_MAXCC 32
+static inline int dd_tdd_tiling_is_runs(unsigned nbits, dd_tdd_tiling t)
+{
+    return nbits == 4 && t.samples_per_lane == DD_TDD_RUNS_SPL;
+}
+
 // The instantiated tilings for nbits (16-channel chunks): writes up to max
 // into out and returns how many there are.
 int dd_tdd_lds_tilings(unsigned nbits, dd_tdd_tiling* out, int max);
@@ -81,9 +106,18 @@ typedef struct {
 // ceil(nsamps_out / (64 * samples_per_lane)).
 hipError_t dd_tdd_dedisperse_lds_batch(const dd_tdd_beam* beams, const uint32_t* tile_beam,
                                        size_t ntiles, unsigned nbits, const uint32_t* code,
-                                       const int32_t* chan, const uint32_t* wstart, size_t nact,
+                                       const int32_t* chan, const uint32_t* wstart,
+                                       const uint32_t* chunks, unsigned maxc, size_t nact,
                                        size_t ndm, dd_tdd_epilogue ep, dd_tdd_tiling tiling,
                                        hipStream_t stream);
+
+// The runs-layout channel sum (see dd_tdd_tiling_is_runs).
+hipError_t dd_tdd_dedisperse_runs(const uint32_t* rows, size_t row_words, const uint32_t* code,
+                                  const int32_t* chan, const uint32_t* wstart, const uint32_t* chunks,
+                                  unsigned maxc, size_t nact, size_t ndm, size_t nsamps_out,
+                                  uint8_t* out, size_t out_stride, dd_tdd_epilogue ep,
+                                  unsigned dms_per_wave, unsigned waves, const dd_tdd_beam* beams,
+                                  const uint32_t* tile_beam, size_t batch_tiles, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -649,6 +649,13 @@ extern "C" hipError_t dd_tdd_dedisperse_generic(const uint32_t* rows, size_t row
 extern "C" int dd_tdd_lds_tilings(unsigned nbits, dd_tdd_tiling* out, int max)
 {
     int n = 0;
+    if (nbits == 4) {
+        // the runs-layout kernel (tdd_runs.hip)
+        if (n < max && out) out[n] = dd_tdd_tiling{DD_TDD_RUNS_SPL, 4, 4, 16, 0};
+        ++n;
+        if (n < max && out) out[n] = dd_tdd_tiling{DD_TDD_RUNS_SPL, 2, 8, 16, 0};
+        ++n;
+    }
 #define DD_LIST(NB, GG, DPW, WAVES)                                        \
     if (nbits == NB) {                                                     \
         if (n < max && out) {                                              \
@@ -663,6 +670,8 @@ extern "C" int dd_tdd_lds_tilings(unsigned nbits, dd_tdd_tiling* out, int max)
 
 extern "C" int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tl)
 {
+    if (dd_tdd_tiling_is_runs(nbits, tl))
+        return (tl.dms_per_wave == 4 && tl.waves == 4) || (tl.dms_per_wave == 2 && tl.waves == 8);
     const unsigned spw = 32 / nbits;
     if (tl.samples_per_lane % spw) return 0;
     const unsigned G = tl.samples_per_lane / spw;
@@ -675,12 +684,17 @@ extern "C" int dd_tdd_lds_supported(unsigned nbits, dd_tdd_tiling tl)
 
 extern "C" hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_words, unsigned nbits,
                                             const uint32_t* code, const int32_t* chan,
-                                            const uint32_t* wstart, size_t nact, size_t ndm,
+                                            const uint32_t* wstart, const uint32_t* chunks, unsigned maxc,
+                                            size_t nact, size_t ndm,
                                             size_t nsamps_out, uint8_t* out, size_t out_stride,
                                             dd_tdd_epilogue ep, dd_tdd_tiling tl,
                                             hipStream_t stream)
 {
     if (nsamps_out == 0 || ndm == 0) return hipSuccess;
+    if (dd_tdd_tiling_is_runs(nbits, tl))
+        return dd_tdd_dedisperse_runs(rows, row_words, code, chan, wstart, chunks, maxc, nact, ndm, nsamps_out,
+                                      out, out_stride, ep, tl.dms_per_wave, tl.waves, nullptr, nullptr, 0,
+                                      stream);
     const unsigned spw = 32 / nbits;
     const unsigned G = tl.samples_per_lane / spw;
     const uint32_t maxv = (1u << nbits) - 1u;
@@ -703,12 +717,16 @@ extern "C" hipError_t dd_tdd_dedisperse_lds(const uint32_t* rows, size_t row_wor
 
 extern "C" hipError_t dd_tdd_dedisperse_lds_batch(const dd_tdd_beam* beams, const uint32_t* tile_beam,
                                                   size_t ntiles, unsigned nbits, const uint32_t* code,
-                                                  const int32_t* chan, const uint32_t* wstart, size_t nact,
+                                                  const int32_t* chan, const uint32_t* wstart,
+                                                  const uint32_t* chunks, unsigned maxc, size_t nact,
                                                   size_t ndm, dd_tdd_epilogue ep, dd_tdd_tiling tl,
                                                   hipStream_t stream)
 {
     if (ntiles == 0 || ndm == 0) return hipSuccess;
     if (!beams || !tile_beam) return hipErrorInvalidValue;
+    if (dd_tdd_tiling_is_runs(nbits, tl))
+        return dd_tdd_dedisperse_runs(nullptr, 0, code, chan, wstart, chunks, maxc, nact, ndm, 0, nullptr, 0, ep,
+                                      tl.dms_per_wave, tl.waves, beams, tile_beam, ntiles, stream);
     const unsigned spw = 32 / nbits;
     const unsigned G = tl.samples_per_lane / spw;
     const uint32_t maxv = (1u << nbits) - 1u;

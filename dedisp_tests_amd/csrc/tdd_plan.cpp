@@ -30,11 +30,12 @@ struct TDDState {
     DeviceBuffer d_chan;    // [nact] channel index
     DeviceBuffer d_weight;  // [nact] killmask weight (generic path)
     DeviceBuffer d_wstart;  // [ndmblk][nact] staged window start (fast path)
+    DeviceBuffer d_chunks;  // [ndmblk][maxc] staging chunks (runs-layout kernel)
 
     // host copies of the tables (what the device buffers above hold once
     // uploaded): fast path delay codes or generic [ndm][nact] delays, channel
     // indices, weights, window starts
-    std::vector<uint32_t> h_tab, h_wstart;
+    std::vector<uint32_t> h_tab, h_wstart, h_chunks;
     std::vector<int32_t> h_chan, h_weight;
     bool uploaded = false;   // d_code / d_chan / d_weight / d_wstart hold the host tables
     uint64_t table_uid = 0;  // process-unique id of the current host tables
@@ -50,6 +51,7 @@ struct TDDState {
     size_t nact = 0;
     size_t dmax = 0;        // largest delay (samples) over the DMs in use
     size_t max_wstart = 0;  // fast path
+    unsigned maxc = 0;      // runs-layout kernel: chunk entries per DM block (terminator included)
     size_t row_words = 0;
 
     // the caller chose the gulp size (set_gulp_size): execute in gulps of it
@@ -120,13 +122,104 @@ dd_tdd_tiling default_tiling(unsigned nbits)
     switch (nbits) {
     case 1: t.samples_per_lane = 32; break;
     case 2: t.samples_per_lane = 16; break;
-    case 4: t.samples_per_lane = 16; t.dms_per_wave = 4; t.chunk_channels = 32; break;
+    // 4-bit: the runs-layout kernel (7 runs x 4 positions per lane, 4 DMs x
+    // 4 waves): 22.6 ms on 1024 ch x 2^20 x 1167 DMs, against 25.8 ms for
+    // the packed-lane kernel's best tiling (16 samples per lane, 4 DMs x 4
+    // waves, 32-channel chunks).
+    case 4: t.samples_per_lane = DD_TDD_RUNS_SPL; t.dms_per_wave = 4; t.chunk_channels = 32; break;
     default: t.samples_per_lane = 16; break;
     }
     return t;
 }
 
 constexpr size_t kLdsBudget = 64 * 1024;  // bytes of LDS per workgroup
+
+// Output samples per time tile of the channel-sum kernel: 7 runs of 256 for
+// the runs-layout kernel, 64 lanes x samples_per_lane for the packed-lane one.
+size_t tile_samples(unsigned nbits, const dd_tdd_tiling& t)
+{
+    return dd_tdd_tiling_is_runs(nbits, t) ? size_t(7 * 256) : size_t(64) * t.samples_per_lane;
+}
+
+// Tables of the runs-layout channel sum (tdd_runs.hip) from the integer delays
+// d[dm][j] (samples): per DM block (the tiling's waves x DMs per wave) and channel, the staged window
+// start (the smallest delay rounded down to 16 samples) and the positions
+// the block's DMs read (T = 256 plus the delay span, in whole 128s); channels
+// are grouped, in order, into chunks whose staged windows and run words fit
+// the kernel's two LDS buffers (at most cap channels each); codes are the
+// LDS byte offsets of every DM's window in its chunk's run words. False when
+// one channel's window does not fit (then the packed-lane kernel is used).
+bool build_runs_tables(const std::vector<uint32_t>& d, size_t dm_count, size_t nact, const dd_tdd_tiling& tl,
+                       std::vector<uint32_t>& code, std::vector<uint32_t>& wstart, std::vector<uint32_t>& chunks,
+                       unsigned& maxc, size_t& max_wstart, size_t& window_words)
+{
+    constexpr size_t T = 256;
+    const size_t DPW = tl.dms_per_wave, WAVES = tl.waves, DPB = DPW * WAVES;
+    unsigned cap = tl.chunk_channels;
+    constexpr size_t L1 = DD_TDD_RUNS_L1_BYTES, L2 = DD_TDD_RUNS_L2_BYTES;
+    if (nact == 0 || nact > 4369 || dm_count == 0) return false;
+    cap = std::max(1u, std::min(cap, (unsigned)DD_TDD_RUNS_MAXCC));
+    auto slot1 = [](size_t P) { return div_up(6 * T / 8 + P / 8, size_t(64)) * 72 * 4; };
+    const size_t ndmblk = div_up(dm_count, DPB);
+    wstart.assign(ndmblk * nact, 0);
+    std::vector<std::vector<uint32_t>> blk(ndmblk);
+    std::vector<uint32_t> chunk_of(ndmblk * nact);  // chunk index within the block
+    max_wstart = 0;
+    window_words = 0;
+    size_t most = 0;
+    for (size_t b = 0; b < ndmblk; ++b) {
+        const size_t i0 = b * DPB, i1 = std::min(dm_count, i0 + DPB);
+        size_t j0 = 0, cc = 0, P = 0;
+        auto close = [&]() {
+            blk[b].push_back((uint32_t)j0 | (uint32_t)cc << 16 | (uint32_t)(P / 128) << 24);
+            window_words = std::max(window_words, 6 * T / 8 + P / 8);
+        };
+        for (size_t j = 0; j < nact; ++j) {
+            uint32_t lo = ~0u, hi = 0;
+            for (size_t i = i0; i < i1; ++i) {
+                lo = std::min(lo, d[i * nact + j]);
+                hi = std::max(hi, d[i * nact + j]);
+            }
+            const uint32_t a = lo & ~15u;
+            wstart[b * nact + j] = a / 8;
+            max_wstart = std::max<size_t>(max_wstart, a / 8);
+            const size_t need = round_up(T + (hi - a), size_t(128));
+            size_t P2 = std::max(P, need);
+            if (cc > 0 && ((cc + 1) * slot1(P2) > L1 || (cc + 1) * P2 * 4 > L2 || cc + 1 > cap)) {
+                close();
+                j0 = j;
+                cc = 0;
+                P2 = need;
+            }
+            if (slot1(P2) > L1 || P2 * 4 > L2 || P2 / 128 > 255) return false;
+            P = P2;
+            chunk_of[b * nact + j] = (uint32_t)blk[b].size();
+            ++cc;
+        }
+        close();
+        most = std::max(most, blk[b].size());
+    }
+    maxc = (unsigned)(most + 1);
+    chunks.assign(ndmblk * maxc, 0);
+    for (size_t b = 0; b < ndmblk; ++b)
+        std::copy(blk[b].begin(), blk[b].end(), chunks.begin() + b * maxc);
+    // code[group][j][i]: DMs past the end repeat the last one
+    const size_t ngroups = ndmblk * WAVES;
+    code.assign(ngroups * nact * DPW, 0);
+    for (size_t g = 0; g < ngroups; ++g) {
+        const size_t b = g / WAVES;
+        for (size_t j = 0; j < nact; ++j) {
+            const uint32_t e = blk[b][chunk_of[b * nact + j]];
+            const size_t cj0 = e & 0xffffu, P = (size_t)(e >> 24) * 128;
+            const uint32_t a = wstart[b * nact + j] * 8;
+            for (size_t i = 0; i < DPW; ++i) {
+                const size_t dm = std::min(g * DPW + i, dm_count - 1);
+                code[(g * nact + j) * DPW + i] = (uint32_t)(((j - cj0) * P + (d[dm * nact + j] - a)) * 4);
+            }
+        }
+    }
+    return true;
+}
 
 }  // namespace
 
@@ -197,9 +290,24 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
         bool fast = binary && nact > 0 && nbits <= 8 && dd_tdd_lds_supported(nbits, tl) &&
                     (uint64_t)dm_count * nact < (uint64_t(1) << 31);
 
-        std::vector<uint32_t> code, wstart;
+        std::vector<uint32_t> code, wstart, chunks;
         size_t max_wstart = 0, W = 0;
-        if (fast) {
+        unsigned maxc = 0;
+        if (fast && dd_tdd_tiling_is_runs(nbits, tl)) {
+            size_t ww = 0;
+            if (build_runs_tables(d, dm_count, nact, tl, code, wstart, chunks, maxc, max_wstart, ww)) {
+                tl.window_words = (unsigned)ww;
+            } else {
+                // a window too long for the runs kernel's LDS buffers
+                tl = dd_tdd_tiling{16, 4, 4, 32, 0};
+                code.clear();
+                wstart.clear();
+                chunks.clear();
+                maxc = 0;
+                max_wstart = 0;
+            }
+        }
+        if (fast && !dd_tdd_tiling_is_runs(nbits, tl)) {
             const size_t G = tl.samples_per_lane / spw;
             const size_t dpb = (size_t)tl.waves * tl.dms_per_wave;
             const size_t ndmblk = div_up(dm_count, dpb);
@@ -255,6 +363,8 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
         st.h_weight = std::move(weight);
         st.h_tab = fast ? std::move(code) : std::move(d);
         st.h_wstart = fast ? std::move(wstart) : std::vector<uint32_t>();
+        st.h_chunks = fast ? std::move(chunks) : std::vector<uint32_t>();
+        st.maxc = fast ? maxc : 0;
         st.uploaded = false;
         st.table_uid = ++g_table_uid;
 
@@ -270,7 +380,9 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
     }
     if (upload && !st.uploaded) {
         const size_t nact = st.nact;
-        st.d_chan.reserve(std::max<size_t>(nact, 1) * 4);
+        // 8 spare entries: the runs-layout kernel reads channel numbers and
+        // window starts in groups of 8
+        st.d_chan.reserve((nact + 8) * 4);
         st.d_weight.reserve(std::max<size_t>(nact, 1) * 4);
         if (nact) {
             DD_HIP(hipMemcpyAsync(st.d_chan.ptr, st.h_chan.data(), nact * 4, hipMemcpyHostToDevice, stream));
@@ -283,9 +395,13 @@ void TDDPlan::prepare_tables(unsigned nbits, dedisp_size first_dm_idx, dedisp_si
             DD_HIP(hipMemcpyAsync(st.d_code.ptr, st.h_tab.data(), st.h_tab.size() * 4, hipMemcpyHostToDevice,
                                   stream));
         if (st.fast) {
-            st.d_wstart.reserve(st.h_wstart.size() * 4);
+            st.d_wstart.reserve((st.h_wstart.size() + 8) * 4);
             DD_HIP(hipMemcpyAsync(st.d_wstart.ptr, st.h_wstart.data(), st.h_wstart.size() * 4,
                                   hipMemcpyHostToDevice, stream));
+            st.d_chunks.reserve(std::max<size_t>(st.h_chunks.size(), 1) * 4);
+            if (!st.h_chunks.empty())
+                DD_HIP(hipMemcpyAsync(st.d_chunks.ptr, st.h_chunks.data(), st.h_chunks.size() * 4,
+                                      hipMemcpyHostToDevice, stream));
         }
         DD_HIP(hipStreamSynchronize(stream));
         st.uploaded = true;
@@ -300,7 +416,7 @@ size_t TDDPlan::row_words_for(unsigned nbits, size_t nsamps, size_t nsc) const
     const size_t spw = 32 / nbits;
     size_t row_words;
     if (st.fast) {
-        const size_t tile_words = (size_t)64 * (st.tiling.samples_per_lane / spw);
+        const size_t tile_words = tile_samples(nbits, st.tiling) / spw;
         const size_t ntiles = div_up(nsc, tile_words * spw);
         row_words = ntiles * tile_words + st.max_wstart + st.tiling.window_words;
     } else {
@@ -479,12 +595,12 @@ void TDDPlan::execute_span(size_type nsamps, const byte_type* in, size_type in_n
         const size_t b0 = k * blk, nb = std::min(blk, dm_count - b0);
         if (st.fast) {
             // the tables of DM block b0 / (waves * dms_per_wave) start at DM b0
+            const size_t b0blk = b0 / ((size_t)st.tiling.waves * st.tiling.dms_per_wave);
             DD_HIP(dd_tdd_dedisperse_lds(st.d_rows.as<uint32_t>(), st.row_words, nbits,
                                          st.d_code.as<uint32_t>() + b0 * st.nact, st.d_chan.as<int32_t>(),
-                                         st.d_wstart.as<uint32_t>() +
-                                             b0 / ((size_t)st.tiling.waves * st.tiling.dms_per_wave) * st.nact,
-                                         st.nact, nb, nsc, d_out + b0 * d_out_stride, d_out_stride, ep, st.tiling,
-                                         stream));
+                                         st.d_wstart.as<uint32_t>() + b0blk * st.nact,
+                                         st.d_chunks.as<uint32_t>() + b0blk * st.maxc, st.maxc, st.nact, nb,
+                                         nsc, d_out + b0 * d_out_stride, d_out_stride, ep, st.tiling, stream));
         } else {
             DD_HIP(dd_tdd_dedisperse_generic(st.d_rows.as<uint32_t>(), st.row_words, nbits,
                                              st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
@@ -569,9 +685,10 @@ void TDDPlan::execute_noalloc(size_type nsamps, const byte_type* in, size_type i
         return at;
     };
     const size_t o_code = carve((st.h_tab.size() + 64) * 4);
-    const size_t o_chan = carve(st.nact * 4);
+    const size_t o_chan = carve((st.nact + 8) * 4);
     const size_t o_weight = carve(st.nact * 4);
-    const size_t o_wstart = carve(st.h_wstart.size() * 4);
+    const size_t o_wstart = carve((st.h_wstart.size() + 8) * 4);
+    const size_t o_chunks = carve(st.h_chunks.size() * 4);
     const size_t o_rows = carve(m_nchans * row_words * 4);
     if (used > scratch_bytes) DD_THROW(DEDISP_MEM_ALLOC_FAILED);
     uint8_t* const base = static_cast<uint8_t*>(scratch);
@@ -579,6 +696,7 @@ void TDDPlan::execute_noalloc(size_type nsamps, const byte_type* in, size_type i
     auto* d_chan = reinterpret_cast<int32_t*>(base + o_chan);
     auto* d_weight = reinterpret_cast<int32_t*>(base + o_weight);
     auto* d_wstart = reinterpret_cast<uint32_t*>(base + o_wstart);
+    auto* d_chunks = reinterpret_cast<uint32_t*>(base + o_chunks);
     auto* d_rows = reinterpret_cast<uint32_t*>(base + o_rows);
 
     // tables: uploaded only when the scratch holds another plan's (or an
@@ -594,6 +712,9 @@ void TDDPlan::execute_noalloc(size_type nsamps, const byte_type* in, size_type i
         if (!st.h_wstart.empty())
             DD_HIP(hipMemcpyAsync(d_wstart, st.h_wstart.data(), st.h_wstart.size() * 4, hipMemcpyHostToDevice,
                                   stream));
+        if (!st.h_chunks.empty())
+            DD_HIP(hipMemcpyAsync(d_chunks, st.h_chunks.data(), st.h_chunks.size() * 4, hipMemcpyHostToDevice,
+                                  stream));
         DD_HIP(hipStreamSynchronize(stream));
         *scratch_tag = st.table_uid;
     }
@@ -605,8 +726,8 @@ void TDDPlan::execute_noalloc(size_type nsamps, const byte_type* in, size_type i
     const size_t out_stride = nsc * (out_nbits / 8);
     const dd_tdd_epilogue ep = make_epilogue(nbits, out_nbits, m_nchans);
     if (st.fast)
-        DD_HIP(dd_tdd_dedisperse_lds(d_rows, row_words, nbits, d_code, d_chan, d_wstart, st.nact, m_dm_count, nsc,
-                                     out, out_stride, ep, st.tiling, stream));
+        DD_HIP(dd_tdd_dedisperse_lds(d_rows, row_words, nbits, d_code, d_chan, d_wstart, d_chunks, st.maxc,
+                                     st.nact, m_dm_count, nsc, out, out_stride, ep, st.tiling, stream));
     else
         DD_HIP(dd_tdd_dedisperse_generic(d_rows, row_words, nbits, d_code, d_chan, d_weight, st.nact, m_dm_count,
                                          nsc, out, out_stride, ep, stream));
@@ -653,7 +774,7 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
     const bool device_ptrs = (flags & DEDISP_DEVICE_POINTERS) != 0;
 
     // ---- layout: rows, staged input/output and time tiles of every beam ----
-    const size_t tile_samps = (size_t)64 * st.tiling.samples_per_lane;
+    const size_t tile_samps = tile_samples(nbits, st.tiling);
     std::vector<size_t> nsc(nbeams), rw(nbeams), roff(nbeams), ioff(nbeams), ooff(nbeams), nt(nbeams);
     size_t rows_total = 0, in_total = 0, out_total = 0, ntiles = 0;
     double work = 0;
@@ -683,6 +804,7 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
                                  st.tiling.samples_per_lane, st.tiling.dms_per_wave, st.tiling.waves,
                                  st.tiling.chunk_channels, st.tiling.window_words,
                                  (uint64_t)st.d_code.ptr, (uint64_t)st.d_chan.ptr, (uint64_t)st.d_wstart.ptr,
+                                 (uint64_t)st.d_chunks.ptr, st.maxc,
                                  (uint64_t)st.d_brows.ptr};
     for (size_t b = 0; b < nbeams; ++b) {
         key.push_back(nsamps[b]);
@@ -750,7 +872,8 @@ void TDDPlan::execute_batch(size_type nbeams, const size_type* nsamps, const byt
         if (timed) ctx.timer.begin(StageTimer::DEDISPERSE, stream);
         DD_HIP(dd_tdd_dedisperse_lds_batch(d_beams.as<dd_tdd_beam>(), d_tile_beam.as<uint32_t>(),
                                            ntiles, nbits, st.d_code.as<uint32_t>(), st.d_chan.as<int32_t>(),
-                                           st.d_wstart.as<uint32_t>(), st.nact, m_dm_count, ep,
+                                           st.d_wstart.as<uint32_t>(), st.d_chunks.as<uint32_t>(), st.maxc,
+                                           st.nact, m_dm_count, ep,
                                            st.tiling, stream));
         if (timed) ctx.timer.end(StageTimer::DEDISPERSE, stream);
     };

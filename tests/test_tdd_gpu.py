@@ -83,6 +83,41 @@ def test_tdd_4bit_tilings(dd, orc, monkeypatch, tiling):
     assert np.array_equal(out, ref)
 
 
+@pytest.mark.parametrize("shape", ["4,4", "2,8"])
+@pytest.mark.parametrize("cap", [64, 3, 1])
+def test_tdd_runs_kernel_vs_oracle(dd, orc, monkeypatch, cap, shape):
+    # The runs-layout kernel (4-bit, samples_per_lane 28; 4 DMs x 4 waves or
+    # 2 DMs x 8 waves): 100 DMs (the last block of 16 partly filled), 512
+    # channels; cap = channels per staged chunk (64: clamped to 32; 3: odd
+    # chunks, byte lanes flushed across chunk boundaries; 1: one channel per
+    # chunk).
+    monkeypatch.setenv("DEDISP_TDD_TILING", f"28,{shape},{cap}")
+    dms = np.arange(0, 90, 0.9, dtype=f32)
+    out, ref = _random_case(dd, orc, 4, 512, 9000, dms, seed=31 + cap, geom=(54.6e-6, 1530.0, -0.625))
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("shape", ["4,4", "2,8"])
+def test_tdd_runs_kernel_wide_windows(dd, orc, monkeypatch, shape):
+    # DM steps of 2.9 at DM ~1000 over 1024 channels 1530 MHz down by 0.3125:
+    # a block's windows span up to ~900 samples, so chunks hold few channels
+    # and the positions per chunk vary along the band.
+    monkeypatch.setenv("DEDISP_TDD_TILING", f"28,{shape},32")
+    dms = np.arange(950.0, 950.0 + 40 * 2.9, 2.9, dtype=f32)
+    out, ref = _random_case(dd, orc, 4, 1024, 26000, dms, seed=41, geom=(54.6e-6, 1530.0, -0.3125))
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("nch,out_nbits", [(38, 8), (130, 16), (202, 32)])
+def test_tdd_runs_kernel_ragged_and_killmask(dd, orc, monkeypatch, nch, out_nbits):
+    monkeypatch.setenv("DEDISP_TDD_TILING", "28,4,4,64")
+    km = np.ones(nch, np.int32)
+    km[::7] = 0
+    dms = np.linspace(0, 45, 19, dtype=f32)
+    out, ref = _random_case(dd, orc, 4, nch, 4000 + nch, dms, seed=50 + nch, killmask=km, out_nbits=out_nbits)
+    assert np.array_equal(out, ref)
+
+
 @pytest.mark.parametrize("tiling", ["8,2,4,64", "16,2,4,64", "8,4,4,64", "8,2,8,32"])
 def test_tdd_8bit_tilings(dd, orc, monkeypatch, tiling):
     monkeypatch.setenv("DEDISP_TDD_TILING", tiling)
