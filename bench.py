@@ -779,8 +779,8 @@ def tdd_leg(torch, dd, steps: int, warmup: int) -> tuple:
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": "profiles/tdd-1024ch-2p20x4bit-1167dm_hbm-traffic.md",
-                     "kernel": "tdd_lds_k", "kernel_ms": k_ms, "algorithmic_bytes": in_bytes + out_bytes,
-                     "note": "integer-add work bounds this kernel, not HBM: see valu_roofline"},
+                     "kernel": "tdd_runs_k", "kernel_ms": k_ms, "algorithmic_bytes": in_bytes + out_bytes,
+                     "note": "LDS reads and VALU issue bound this kernel, not HBM: see valu_roofline"},
         "valu_roofline": valu_roofline(k_ms, adds_t),
         "stages_ms": {"transpose": float(np.mean(unpack_ms)), "channel_sum": k_ms},
         "parity": parity,
