@@ -327,9 +327,10 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
     const int tq = hi ? (base ? T3 - base : T3 / 2) : base;
     const bool self0 = (tq == 0), selfh = (tq == T3 / 2);
     float2 q[A];
-    if (!period && nbits == 8 && t0 + 2 * (size_t)M <= nvalid) {
-        // interior 8-bit chunk: every pair in bounds, no branch between the
-        // loads, so all of a thread's loads are in flight together
+    if (nbits == 8 && t0 + 2 * (size_t)M <= nvalid && (!period || t0 + 2 * (size_t)M <= period)) {
+        // interior 8-bit chunk: every pair in bounds (and, for a periodic
+        // series, before the wrap, where t mod period = t), no branch between
+        // the loads, so all of a thread's loads are in flight together
         auto interior = [&](int n) -> float2 {
             const uint16_t pair = *reinterpret_cast<const uint16_t*>(src + t0 + 2 * (size_t)n);
             return make_float2(((float)(pair & 0xff) - mid) * s, ((float)(pair >> 8) - mid) * s);
