@@ -22,6 +22,8 @@
 // double precision on the host: W_M^j = conj(ph[2j]).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 
 #include "fdd_kernels.h"
@@ -1093,7 +1095,187 @@ __global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict
     out[row * out_stride + k] = make_float2(e.x + r.x, e.y + r.y);
 }
 
+// Steps B and C in one pass for even m <= 72. A tile is columns A = 64w + l
+// and their mirrors B = 8192 - A (l < 64, all m rows) of one row: the
+// workgroup stages them in LDS, and each of its 4 waves sums BC_KH of the
+// radix-2 outputs k (E[k] / O[k] over the even / odd rows, as step B) for
+// column A and, at h-1-k, for column B, then pairs them. The real-input
+// pairing of bin kk = k2 + 8192 k1 takes Z[L - kk], which for k2 = A >= 1 is
+// row m-1-k1 of column B: rows k and k+h of A pair with rows 2h-1-k and h-1-k
+// of B, and the other way round. Column 4096 is its own mirror (B = A; its B
+// outputs are not written again); column 0 is paired within itself by
+// fourier_col0_k. Each column is read from HBM once, and the intermediate Z
+// never goes to HBM. The grid is persistent (tiles t = blockIdx.x + i
+// gridDim.x, w = t % 65, row = t / 65): the next tile's loads are in flight
+// in registers while the current one is summed.
+constexpr int BC_KH = 9;                    // outputs k per wave: h = m/2 <= 36
+constexpr int BC_MAXM = 8 * BC_KH;          // 72
+constexpr int BC_LDS = 128 + BC_MAXM * 128;  // float2: W_m^e, then [row][128] column values
+constexpr int BC_TILES_PER_ROW = 65;
+
+__global__ __launch_bounds__(256) void fourier_step_bc_k(const float2* __restrict__ A, size_t a_stride,
+                                                         size_t m, uint32_t nrows,
+                                                         const float2* __restrict__ twL,
+                                                         const float2* __restrict__ twN, float2* __restrict__ out,
+                                                         size_t out_stride)
+{
+    __shared__ float2 bc_lds[BC_LDS];
+    float2* const wt = bc_lds;         // W_m^e, e < m
+    float2* const col = bc_lds + 128;  // [row][128]: lane l's column A at l, B at 64 + l
+    const uint32_t h = (uint32_t)(m / 2);
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t kb = wave * BC_KH;
+    const uint32_t ntile = BC_TILES_PER_ROW * nrows;
+    for (uint32_t i = t; i < m; i += 256) wt[i] = twL[(size_t)i * 8192];
+    // staging: thread t holds tile column c = t & 127 at rows 2i + (t >> 7),
+    // i < BC_MAXM/2, the rows past m - 1 clamped to m - 2 + (t >> 7) (loaded
+    // and stored, never summed); columns clamped into [1, 4096] (the lanes
+    // past them compute but do not write). Byte offsets from the row base fit
+    // 32 bits (< 72 x 8192 x 8)
+    const uint32_t sc = t & 127, sr0 = t >> 7;
+    float2 v[BC_MAXM / 2];
+    auto load = [&](uint32_t tile) {
+        const uint32_t w = tile % BC_TILES_PER_ROW, row = tile / BC_TILES_PER_ROW;
+        const uint32_t ca = min(max(64 * w + (sc & 63), 1u), 4096u);
+        const char* base = reinterpret_cast<const char*>(A + row * a_stride);
+        const uint32_t off0 = (sr0 * 8192 + ((sc < 64) ? ca : 8192 - ca)) * 8;
+        // opaque copy of the last even row: keeps the 36 clamped offsets from
+        // being hoisted out of the tile loop into as many scalar registers
+        uint32_t rlast = 2 * h - 2;
+        asm volatile("" : "+s"(rlast));
+#pragma unroll
+        for (int i = 0; i < BC_MAXM / 2; ++i) {
+            const uint32_t r2 = min((uint32_t)(2 * i), rlast);
+            v[i] = *reinterpret_cast<const float2*>(base + off0 + r2 * (8192 * 8));
+        }
+    };
+    // pairing of bin kk with Z[L - kk] (the step-C formula)
+    auto pair_out = [&](float2* o_row, size_t kk, float2 za, float2 zb) {
+        const float2 ev = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+        const float2 d = make_float2(za.x - zb.x, za.y + zb.y);
+        const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+        const float2 r = cmul(o, twN[kk]);
+        o_row[kk] = make_float2(ev.x + r.x, ev.y + r.y);
+    };
+    uint32_t tile = blockIdx.x;
+    if (tile < ntile) load(tile);
+    for (; tile < ntile; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's sums have read col (and wt is written)
+#pragma unroll
+        for (int i = 0; i < BC_MAXM / 2; ++i) col[(2 * i + sr0) * 128 + sc] = v[i];
+        __syncthreads();
+        if (tile + gridDim.x < ntile) load(tile + gridDim.x);
+        if (kb < h) {
+            // all BC_KH outputs are summed (the ones past h - 1 are not written):
+            // no branch in the loop, and e stays in scalar registers
+            float2 EA[BC_KH], OA[BC_KH], EB[BC_KH], OB[BC_KH];
+            uint32_t e[BC_KH + 1];  // (p * (kb + j)) mod h, j <= BC_KH
+#pragma unroll
+            for (int j = 0; j < BC_KH; ++j) EA[j] = OA[j] = EB[j] = OB[j] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j <= BC_KH; ++j) e[j] = 0;
+            for (uint32_t p = 0; p < h; ++p) {
+                const float2 ae = col[(2 * p) * 128 + lane], ao = col[(2 * p + 1) * 128 + lane];
+                const float2 be = col[(2 * p) * 128 + 64 + lane], bo = col[(2 * p + 1) * 128 + 64 + lane];
+#pragma unroll
+                for (int j = 0; j < BC_KH; ++j) {
+                    // A at k = kb + j: W_h^{p k}; B at h-1-k: W_h^{p (h-1-k)} = conj W_h^{p (k+1)}
+                    const float2 wa = wt[2 * e[j]], wb0 = wt[2 * e[j + 1]];
+                    const float2 wb = make_float2(wb0.x, -wb0.y);
+                    EA[j].x = fmaf(ae.x, wa.x, fmaf(-ae.y, wa.y, EA[j].x));
+                    EA[j].y = fmaf(ae.x, wa.y, fmaf(ae.y, wa.x, EA[j].y));
+                    OA[j].x = fmaf(ao.x, wa.x, fmaf(-ao.y, wa.y, OA[j].x));
+                    OA[j].y = fmaf(ao.x, wa.y, fmaf(ao.y, wa.x, OA[j].y));
+                    EB[j].x = fmaf(be.x, wb.x, fmaf(-be.y, wb.y, EB[j].x));
+                    EB[j].y = fmaf(be.x, wb.y, fmaf(be.y, wb.x, EB[j].y));
+                    OB[j].x = fmaf(bo.x, wb.x, fmaf(-bo.y, wb.y, OB[j].x));
+                    OB[j].y = fmaf(bo.x, wb.y, fmaf(bo.y, wb.x, OB[j].y));
+                }
+#pragma unroll
+                for (int j = 0; j <= BC_KH; ++j) {
+                    // x mod h for x < 2h as a minimum (x - h wraps when x < h):
+                    // selects in scalar registers, where a compare-and-subtract
+                    // was lowered to vector instructions
+                    const uint32_t x = e[j] + kb + j;
+                    e[j] = min(x, x - h);
+                }
+            }
+            const uint32_t w = tile % BC_TILES_PER_ROW, row = tile / BC_TILES_PER_ROW;
+            const uint32_t cA = 64 * w + lane, cB = 8192 - cA;
+            if (cA >= 1 && cA <= 4096) {
+                float2* o_row = out + row * out_stride;
+#pragma unroll
+                for (int j = 0; j < BC_KH; ++j) {
+                    const uint32_t k = kb + j;
+                    if (k < h) {
+                        const uint32_t kB = h - 1 - k;
+                        const float2 ta = cmul(OA[j], wt[k]), tb = cmul(OB[j], wt[kB]);
+                        const float2 zA0 = cadd(EA[j], ta), zA1 = csub(EA[j], ta);  // rows k, k+h of A
+                        const float2 zB0 = cadd(EB[j], tb), zB1 = csub(EB[j], tb);  // rows h-1-k, 2h-1-k of B
+                        pair_out(o_row, (size_t)cA + 8192 * (size_t)k, zA0, zB1);
+                        pair_out(o_row, (size_t)cA + 8192 * (size_t)(k + h), zA1, zB0);
+                        if (cA < 4096) {
+                            pair_out(o_row, (size_t)cB + 8192 * (size_t)kB, zB0, zA1);
+                            pair_out(o_row, (size_t)cB + 8192 * (size_t)(kB + h), zB1, zA0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Column 0 of steps B and C: the m-point DFT of the column as direct sums,
+// then row k1 paired with row (m - k1) mod m.
+__global__ __launch_bounds__(128) void fourier_col0_k(const float2* __restrict__ A, size_t a_stride, size_t m,
+                                                      const float2* __restrict__ twL, const float2* __restrict__ twN,
+                                                      float2* __restrict__ out, size_t out_stride)
+{
+    __shared__ float2 zs[128], wt[128], av[128];
+    const size_t row = blockIdx.x;
+    const uint32_t k1 = threadIdx.x;
+    if (k1 < m) {
+        wt[k1] = twL[(size_t)k1 * 8192];
+        av[k1] = A[row * a_stride + (size_t)k1 * 8192];
+    }
+    __syncthreads();
+    if (k1 < m) {
+        float2 acc = make_float2(0.f, 0.f);
+        uint32_t e = 0;
+        for (uint32_t n1 = 0; n1 < m; ++n1) {
+            const float2 v = av[n1], tw = wt[e];
+            acc.x = fmaf(v.x, tw.x, fmaf(-v.y, tw.y, acc.x));
+            acc.y = fmaf(v.x, tw.y, fmaf(v.y, tw.x, acc.y));
+            e += k1;
+            if (e >= m) e -= (uint32_t)m;
+        }
+        zs[k1] = acc;
+    }
+    __syncthreads();
+    if (k1 < m) {
+        const float2 za = zs[k1], zb = zs[k1 ? m - k1 : 0];
+        const size_t kk = 8192 * (size_t)k1;
+        const float2 ev = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+        const float2 d = make_float2(za.x - zb.x, za.y + zb.y);
+        const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+        const float2 r = cmul(o, twN[kk]);
+        out[row * out_stride + kk] = make_float2(ev.x + r.x, ev.y + r.y);
+    }
+}
+
 }  // namespace
+
+// Whether steps B and C run fused: even m <= 72, whose h = m/2 outputs the 4
+// waves of fourier_step_bc_k cover (DEDISP_FDD_BC=0 keeps the two passes).
+inline bool bc_fused(size_t m)
+{
+    static const bool off = [] {
+        const char* v = std::getenv("DEDISP_FDD_BC");
+        return v && v[0] == '0';
+    }();
+    return !off && m % 2 == 0 && m <= (size_t)BC_MAXM;
+}
 
 extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t nrows, size_t N,
                                        const float2* ph16384, const float2* twL, const float2* twN,
@@ -1121,6 +1303,25 @@ extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t n
                            twL, Sr, sp);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+        if (bc_fused(m)) {
+            // steps B and C in one pass (column 0 on its own)
+            // persistent: two workgroups per CU (the LDS tile allows two)
+            int dev = 0, ncu = 0;
+            e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e != hipSuccess) return e;
+            const size_t ntile = (size_t)BC_TILES_PER_ROW * nr;
+            const unsigned grid = (unsigned)std::min<size_t>(ntile, 2 * (size_t)std::max(ncu, 1));
+            hipLaunchKernelGGL(fourier_step_bc_k, dim3(grid), dim3(256), 0, stream, Sr, sp, m, (uint32_t)nr, twL,
+                               twN, out + r0 * out_stride, out_stride);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(fourier_col0_k, dim3(nr), dim3(128), 0, stream, Sr, sp, m, twL, twN,
+                               out + r0 * out_stride, out_stride);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            continue;
+        }
         hipLaunchKernelGGL(fourier_step_b_k, dim3(32, nr), dim3(256), 0, stream, Sr, sp, m, twL, Wr, L);
         e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -211,11 +211,14 @@ def test_fdd_fourier_golden(dd):
     assert rel_l2(out, g["fourier"]) < REL_FP32
 
 
-@pytest.mark.parametrize("nsamps,nch", [(40000, 64), (65536, 256), (2 ** 19, 1024), (1090000, 16)])
+@pytest.mark.parametrize("nsamps,nch", [(40000, 64), (65536, 256), (2 ** 19, 1024), (1090000, 16),
+                                         (16000, 16), (1160000, 16), (1200000, 16)])
 def test_fdd_fourier_vs_oracle(dd, orc, nsamps, nch):
     """The whole-series Fourier output (the fork's default) against the oracle;
-    the row transforms' m-point step (N = 16384 m) runs for odd m (3, 67: the
-    direct DFT, 67 prime) and even m (34: one radix-2 split)."""
+    the row transforms' m-point step (N = 16384 m) runs for odd m (3, 5, 67: the
+    direct DFT, 67 prime), for even m <= 72 fused with the real-input pairing
+    (2, 34, 72: one radix-2 split, mirror columns paired in LDS) and for even
+    m > 72 as two passes (74)."""
     rng = np.random.default_rng(nsamps)
     dt, f0, df = 54.6e-6, 1530.0, -0.3125 * 1024 / nch
     data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch)).astype(np.float32)), 0, 255).astype(np.uint8)
