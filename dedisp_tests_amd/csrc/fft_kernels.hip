@@ -30,6 +30,11 @@
 
 namespace {
 
+// Fourth word of a raw (unswizzled, stride 0) buffer resource on gfx9-family
+// GPUs: 32-bit data format. Accesses at byte offsets at or past the record
+// count are dropped (stores) or read as zero (loads).
+constexpr int kRawBufferWord3 = 0x00020000;
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -454,7 +459,14 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
         };
         fft3_core<A, B, C, +1>(ph, lds, zload, q, reverse, tp);
     } else if constexpr (PACKED) {
-        auto zload = [&](int k) -> float2 { return src[(uint32_t)k]; };
+        // an opaque 32-bit byte offset from the uniform row base (one offset
+        // register per load; k < M, so 8k fits)
+        const char* const srcb = reinterpret_cast<const char*>(src);
+        auto zload = [&](int k) -> float2 {
+            uint32_t off = (uint32_t)k * 8u;
+            asm volatile("" : "+v"(off));
+            return *reinterpret_cast<const float2*>(srcb + off);
+        };
         fft3_core<A, B, C, +1>(ph, lds, zload, q);
     } else {
     const float ymx = src[M].x;  // Y[M]: its imaginary part is not used
@@ -471,58 +483,82 @@ __global__ __launch_bounds__(NT) void fdd_inverse_k(
     // e^{+2 pi i k/nfft} for k = tp + T1 n2 is e^{+2 pi i tp/nfft} e^{i pi n2/C}:
     // one table read per thread, the second factor a constant per element
     const float2 ptp = ph[tp];
-    auto zu = [&](int n2, float2 a, float2 ym) -> float2 {  // ym = Y[M-k]
-        const int k = tp + T1 * n2;
-        float2 b = conjf2(ym);
-        if (k == 0) a.y = 0.f, b = make_float2(ymx, 0.f);
-        const float2 e = cadd(a, b), d = csub(a, b);
+    auto wconst = [](int n2) -> float2 {
         const double ang = M_PI * (double)n2 / (double)C;
-        const float2 w = make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
-        const float2 o = cmul(d, n2 ? cmul(ptp, w) : ptp);
+        return make_float2((float)__builtin_cos(ang), (float)__builtin_sin(ang));
+    };
+    // Zu from a = Y[k], ym = Y[M-k] and the twiddle e^{+2 pi i k/nfft}
+    auto zu = [&](float2 a, float2 ym, float2 tw) -> float2 {
+        const float2 b = conjf2(ym);
+        const float2 e = cadd(a, b), d = csub(a, b);
+        const float2 o = cmul(d, tw);
         return make_float2(e.x - o.y, e.y + o.x);
     };
-    // a 32-bit offset from the workgroup-uniform row base (the scalar-base
-    // load form: one offset register per load instead of a 64-bit address)
-    auto load = [&](int k) -> float2 { return src[(uint32_t)k]; };
+    // a 32-bit byte offset from the workgroup-uniform row base (the
+    // scalar-base load form: one offset register per load instead of a
+    // 64-bit address; k <= M, so 8k fits). The offset is opaque to the
+    // compiler, which would otherwise fold the per-thread part into a 64-bit
+    // address and add each element's constant to it (two registers a load)
+    const char* const srcb = reinterpret_cast<const char*>(src);
+    auto load = [&](int k) -> float2 {
+        uint32_t off = (uint32_t)k * 8u;
+        asm volatile("" : "+v"(off));
+        return *reinterpret_cast<const float2*>(srcb + off);
+    };
     auto pair = [&](float2 (&v)[C], bool act) {
         if (!act) return;
-        if (self0) {
+        // Mirrors come from the partner lane i ^ 32 (element m2 for n2,
+        // element n2 for m2) by v_permlane32_swap, except in wave 0, which
+        // holds the two self-paired indices: 0 (lane 0: element n2 pairs with
+        // its own element C - n2, n2 = 0 with Y[M]) and T1/2 (lane 32: its own
+        // element C-1-n2). Wave 0 stages its values in LDS ([element][lane];
+        // free before pass 1) and takes every mirror from there. The loop has
+        // no branch: all waves read both sources and select with the
+        // wave-uniform w0 (a branch per source kept both paths' values live)
+        const bool w0 = __builtin_amdgcn_readfirstlane(t >> 6) == 0;
+        const int lane = t & 63;
+        static_assert(S::LDS_FLOATS >= 2 * 64 * C, "wave 0 stages its values");
+        if (w0) {
 #pragma unroll
-            for (int e = 0; e < C; ++e) lds[e] = v[e].x, lds[C + e] = v[e].y;
+            for (int e = 0; e < C; ++e) lds[e * 64 + lane] = v[e].x, lds[(C + e) * 64 + lane] = v[e].y;
         }
+        const int sl = (self0 || selfh) ? lane : (lane ^ 32);
+        auto staged = [&](int n2) -> float2 {  // wave 0: the mirror of element n2
+            const int se = self0 ? (C - n2) % C : C - 1 - n2;
+            return make_float2(lds[se * 64 + sl], lds[(C + se) * 64 + sl]);
+        };
 #pragma unroll
         for (int n2 = 0; n2 < C / 2; ++n2) {
             const int m2 = C - 1 - n2;
-            const float2 a1 = v[n2], a2 = v[m2];
-            // mirrors: element m2 of the partner for n2, element n2 for m2
+            float2 a1 = v[n2];
+            const float2 a2 = v[m2];
             float2 b1 = make_float2(other_half(a2.x), other_half(a2.y));
             float2 b2 = make_float2(other_half(a1.x), other_half(a1.y));
-            if (selfh) b1 = a2, b2 = a1;
-            v[n2] = zu(n2, a1, b1);
-            v[m2] = zu(m2, a2, b2);
+            const float2 s1 = staged(n2), s2 = staged(m2);
+            b1 = w0 ? s1 : b1;
+            b2 = w0 ? s2 : b2;
+            if (n2 == 0 && self0) a1.y = 0.f, b1 = make_float2(ymx, 0.f);
+            v[n2] = zu(a1, b1, n2 ? cmul(ptp, wconst(n2)) : ptp);
+            v[m2] = zu(a2, b2, cmul(ptp, wconst(m2)));
             __builtin_amdgcn_sched_barrier(0);  // one pair's temporaries at a time
-        }
-        if (self0) {  // index 0 (one lane): its own element C-n2, from the staged copy
-#pragma unroll
-            for (int n2 = 0; n2 < C; ++n2) {
-                const float2 a = make_float2(lds[n2], lds[C + n2]);
-                const float2 b = n2 ? make_float2(lds[C - n2], lds[2 * C - n2]) : a;
-                v[n2] = zu(n2, a, b);
-            }
         }
     };
     fft3_core<A, B, C, +1>(ph, lds, load, q, pair, tp);
     }
     if (t < S::T3) {
         // one workgroup-uniform base and limit (t0 < nsamps_out for every
-        // chunk processed); per sample a 32-bit offset
+        // chunk processed). The limit is the record count of a raw buffer
+        // resource: the hardware drops stores at byte offsets past it, so the
+        // stores need no branch (a branch per sample split the pass-3 sums
+        // into small blocks and kept their partial values live across them)
         float* const base = dst + t0;
         const uint32_t lim = (uint32_t)min(good, nsamps_out - t0);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(lim * 4u), kRawBufferWord3);
 #pragma unroll
         for (int k0 = 0; k0 < A; ++k0) {
             const uint32_t ta = 2u * ((uint32_t)t + (uint32_t)S::T3 * k0);
-            if (ta < lim) base[ta] = q[k0].x * inv_n;
-            if (ta + 1 < lim) base[ta + 1] = q[k0].y * inv_n;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, q[k0].x * inv_n), rs, (int)(4u * ta), 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, q[k0].y * inv_n), rs, (int)(4u * ta + 4u), 0, 0);
         }
     }
 }
@@ -870,11 +906,14 @@ __global__ __launch_bounds__(1024) void combine_inv_k(const float2* __restrict__
     const uint32_t lim = (uint32_t)min(good, nsamps_out - t0);
     const float inv_n = 1.0f / (float)(2 * R * LBIG);
     const int t = threadIdx.x;
+    // stores bounded by the record count of a raw buffer resource (dropped
+    // past lim by the hardware), with no branch per sample
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(lim * 4u), kRawBufferWord3);
 #pragma unroll
     for (int k0 = 0; k0 < 32; ++k0) {
         const uint32_t ta = 2u * (r + (uint32_t)R * ((uint32_t)t + (uint32_t)S::T3 * k0));
-        if (ta < lim) base[ta] = q[k0].x * inv_n;
-        if (ta + 1 < lim) base[ta + 1] = q[k0].y * inv_n;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, q[k0].x * inv_n), rs, (int)(4u * ta), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, q[k0].y * inv_n), rs, (int)(4u * ta + 4u), 0, 0);
     }
 }
 
