@@ -1028,8 +1028,29 @@ __global__ __launch_bounds__(512) void fourier_step_a_k(
     const size_t n1 = lin % m, row = lin / m;
     const float2* src = T + row * t_stride + n1 * 8192;  // column n1, contiguous
     float2* dst = A + row * a_stride + n1 * 8192;
-    auto load = [&](int n2) -> float2 { return src[n2]; };
-    auto store = [&](int k2, float2 v) { dst[k2] = cmul(v, twL[n1 * (size_t)k2]); };
+    // W_L^{n1 k2} for k2 = 64 h + l as W_L^{64 n1 h} W_L^{n1 l}: two exact
+    // table values per workgroup in LDS (both indices < L) instead of a
+    // gather from the L-entry table per output; written before the engine's
+    // first barrier, read after its last
+    __shared__ float2 tw_hi[128], tw_lo[64];
+    const int t = threadIdx.x;
+    if (t < 128) tw_hi[t] = twL[64 * n1 * (size_t)t];
+    else if (t < 192) tw_lo[t - 128] = twL[n1 * (size_t)(t - 128)];
+    // loads and stores take a 32-bit byte offset from the uniform column base
+    // (opaque, so that it is not folded into a 64-bit address per access)
+    const char* const srcb = reinterpret_cast<const char*>(src);
+    auto load = [&](int n2) -> float2 {
+        uint32_t off = (uint32_t)n2 * 8u;
+        asm volatile("" : "+v"(off));
+        return *reinterpret_cast<const float2*>(srcb + off);
+    };
+    char* const dstb = reinterpret_cast<char*>(dst);
+    auto store = [&](int k2, float2 v) {
+        const float2 w = cmul(tw_hi[k2 >> 6], tw_lo[k2 & 63]);
+        uint32_t off = (uint32_t)k2 * 8u;
+        asm volatile("" : "+v"(off));
+        *reinterpret_cast<float2*>(dstb + off) = cmul(v, w);
+    };
     fft3<16, 16, 32, -1>(ph8192, lds, load, store);
 }
 
