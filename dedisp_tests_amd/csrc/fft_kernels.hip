@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict
 
 // Steps B and C in one pass for even m <= 72. A tile is columns A = 64w + l
 // and their mirrors B = 8192 - A (l < 64, all m rows) of one row: the
-// workgroup stages them in LDS, and each of its 4 waves sums BC_KH of the
+// workgroup stages them in LDS, and each of its BC_WAVES waves sums BC_KH of the
 // radix-2 outputs k (E[k] / O[k] over the even / odd rows, as step B) for
 // column A and, at h-1-k, for column B, then pairs them. The real-input
 // pairing of bin kk = k2 + 8192 k1 takes Z[L - kk], which for k2 = A >= 1 is
@@ -1168,62 +1168,72 @@ __global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict
 // never goes to HBM. The grid is persistent (tiles t = blockIdx.x + i
 // gridDim.x, w = t % 65, row = t / 65): the next tile's loads are in flight
 // in registers while the current one is summed.
-constexpr int BC_KH = 9;                    // outputs k per wave: h = m/2 <= 36
-constexpr int BC_MAXM = 8 * BC_KH;          // 72
-constexpr int BC_LDS = 128 + BC_MAXM * 128;  // float2: W_m^e, then [row][128] column values
+constexpr int BC_WAVES = 8;
+constexpr int BC_KH = 5;                     // outputs k per wave: h = m/2 <= BC_WAVES * BC_KH
+constexpr int BC_MAXM = 72;                  // LDS of two workgroups per CU
+constexpr int BC_NT = BC_WAVES * 64;
+constexpr int BC_RPT = BC_MAXM * 128 / BC_NT;  // staged values per thread
+constexpr int BC_LDS = 128 + BC_MAXM + BC_MAXM * 128;  // float2: W_m^e, W_N^{8192 k1}, [row][128] column values
 constexpr int BC_TILES_PER_ROW = 65;
+static_assert(BC_MAXM / 2 <= BC_WAVES * BC_KH && BC_NT % 128 == 0 && BC_MAXM % (BC_NT / 128) == 0,
+              "the waves cover h = m/2 outputs, the threads the staged rows");
 
-__global__ __launch_bounds__(256) void fourier_step_bc_k(const float2* __restrict__ A, size_t a_stride,
+__global__ __launch_bounds__(BC_NT) void fourier_step_bc_k(const float2* __restrict__ A, size_t a_stride,
                                                          size_t m, uint32_t nrows,
                                                          const float2* __restrict__ twL,
                                                          const float2* __restrict__ twN, float2* __restrict__ out,
                                                          size_t out_stride)
 {
     __shared__ float2 bc_lds[BC_LDS];
-    float2* const wt = bc_lds;         // W_m^e, e < m
-    float2* const col = bc_lds + 128;  // [row][128]: lane l's column A at l, B at 64 + l
+    float2* const wt = bc_lds;                   // W_m^e, e < m
+    float2* const wk = bc_lds + 128;             // W_N^{8192 k1}, k1 < m
+    float2* const col = bc_lds + 128 + BC_MAXM;  // [row][128]: lane l's column A at l, B at 64 + l
     const uint32_t h = (uint32_t)(m / 2);
     const int t = threadIdx.x, lane = t & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const uint32_t kb = wave * BC_KH;
     const uint32_t ntile = BC_TILES_PER_ROW * nrows;
-    for (uint32_t i = t; i < m; i += 256) wt[i] = twL[(size_t)i * 8192];
-    // staging: thread t holds tile column c = t & 127 at rows 2i + (t >> 7),
-    // i < BC_MAXM/2, the rows past m - 1 clamped to m - 2 + (t >> 7) (loaded
-    // and stored, never summed); columns clamped into [1, 4096] (the lanes
-    // past them compute but do not write). Byte offsets from the row base fit
-    // 32 bits (< 72 x 8192 x 8)
+    for (uint32_t i = t; i < m; i += BC_NT) wt[i] = twL[(size_t)i * 8192], wk[i] = twN[(size_t)i * 8192];
+    // staging: thread t holds tile column c = t & 127 at rows RS i + (t >> 7)
+    // (RS = BC_NT / 128 rows a step), i < BC_RPT, the rows past m - 1 clamped
+    // to m - 1 (loaded and stored, never summed); columns clamped into
+    // [1, 4096] (the lanes past them compute but do not write). Byte offsets
+    // from the row base fit 32 bits (< 72 x 8192 x 8)
+    constexpr uint32_t RS = BC_NT / 128;
     const uint32_t sc = t & 127, sr0 = t >> 7;
-    float2 v[BC_MAXM / 2];
+    float2 v[BC_RPT];
     auto load = [&](uint32_t tile) {
         const uint32_t w = tile % BC_TILES_PER_ROW, row = tile / BC_TILES_PER_ROW;
         const uint32_t ca = min(max(64 * w + (sc & 63), 1u), 4096u);
         const char* base = reinterpret_cast<const char*>(A + row * a_stride);
-        const uint32_t off0 = (sr0 * 8192 + ((sc < 64) ? ca : 8192 - ca)) * 8;
-        // opaque copy of the last even row: keeps the 36 clamped offsets from
-        // being hoisted out of the tile loop into as many scalar registers
-        uint32_t rlast = 2 * h - 2;
+        const uint32_t off0 = ((sc < 64) ? ca : 8192 - ca) * 8;
+        // opaque copy of the last row: keeps the clamped offsets from being
+        // hoisted out of the tile loop into as many scalar registers
+        uint32_t rlast = 2 * h - 1;
         asm volatile("" : "+s"(rlast));
 #pragma unroll
-        for (int i = 0; i < BC_MAXM / 2; ++i) {
-            const uint32_t r2 = min((uint32_t)(2 * i), rlast);
-            v[i] = *reinterpret_cast<const float2*>(base + off0 + r2 * (8192 * 8));
+        for (int i = 0; i < BC_RPT; ++i) {
+            const uint32_t r = min((uint32_t)(RS * i) + sr0, rlast);
+            v[i] = *reinterpret_cast<const float2*>(base + off0 + r * (8192 * 8));
         }
     };
-    // pairing of bin kk with Z[L - kk] (the step-C formula)
-    auto pair_out = [&](float2* o_row, size_t kk, float2 za, float2 zb) {
+    // pairing of bin kk = k2 + 8192 k1 with Z[L - kk] (the step-C formula);
+    // W_N^kk = W_N^{k2} W_N^{8192 k1}: the lane's column factor times an LDS
+    // entry (two exact table values) rather than a gather from the L-entry
+    // table per output
+    auto pair_out = [&](float2* o_row, uint32_t k2, uint32_t k1, float2 wc, float2 za, float2 zb) {
         const float2 ev = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
         const float2 d = make_float2(za.x - zb.x, za.y + zb.y);
         const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
-        const float2 r = cmul(o, twN[kk]);
-        o_row[kk] = make_float2(ev.x + r.x, ev.y + r.y);
+        const float2 r = cmul(o, cmul(wc, wk[k1]));
+        o_row[k2 + 8192 * (size_t)k1] = make_float2(ev.x + r.x, ev.y + r.y);
     };
     uint32_t tile = blockIdx.x;
     if (tile < ntile) load(tile);
     for (; tile < ntile; tile += gridDim.x) {
         __syncthreads();  // the previous tile's sums have read col (and wt is written)
 #pragma unroll
-        for (int i = 0; i < BC_MAXM / 2; ++i) col[(2 * i + sr0) * 128 + sc] = v[i];
+        for (int i = 0; i < BC_RPT; ++i) col[(RS * i + sr0) * 128 + sc] = v[i];
         __syncthreads();
         if (tile + gridDim.x < ntile) load(tile + gridDim.x);
         if (kb < h) {
@@ -1265,6 +1275,7 @@ __global__ __launch_bounds__(256) void fourier_step_bc_k(const float2* __restric
             const uint32_t cA = 64 * w + lane, cB = 8192 - cA;
             if (cA >= 1 && cA <= 4096) {
                 float2* o_row = out + row * out_stride;
+                const float2 wA = twN[cA], wB = twN[cB];
 #pragma unroll
                 for (int j = 0; j < BC_KH; ++j) {
                     const uint32_t k = kb + j;
@@ -1273,11 +1284,11 @@ __global__ __launch_bounds__(256) void fourier_step_bc_k(const float2* __restric
                         const float2 ta = cmul(OA[j], wt[k]), tb = cmul(OB[j], wt[kB]);
                         const float2 zA0 = cadd(EA[j], ta), zA1 = csub(EA[j], ta);  // rows k, k+h of A
                         const float2 zB0 = cadd(EB[j], tb), zB1 = csub(EB[j], tb);  // rows h-1-k, 2h-1-k of B
-                        pair_out(o_row, (size_t)cA + 8192 * (size_t)k, zA0, zB1);
-                        pair_out(o_row, (size_t)cA + 8192 * (size_t)(k + h), zA1, zB0);
+                        pair_out(o_row, cA, k, wA, zA0, zB1);
+                        pair_out(o_row, cA, k + h, wA, zA1, zB0);
                         if (cA < 4096) {
-                            pair_out(o_row, (size_t)cB + 8192 * (size_t)kB, zB0, zA1);
-                            pair_out(o_row, (size_t)cB + 8192 * (size_t)(kB + h), zB1, zA0);
+                            pair_out(o_row, cB, kB, wB, zB0, zA1);
+                            pair_out(o_row, cB, kB + h, wB, zB1, zA0);
                         }
                     }
                 }
@@ -1326,7 +1337,7 @@ __global__ __launch_bounds__(128) void fourier_col0_k(const float2* __restrict__
 
 }  // namespace
 
-// Whether steps B and C run fused: even m <= 72, whose h = m/2 outputs the 4
+// Whether steps B and C run fused: even m <= 72, whose h = m/2 outputs the
 // waves of fourier_step_bc_k cover (DEDISP_FDD_BC=0 keeps the two passes).
 inline bool bc_fused(size_t m)
 {
@@ -1372,7 +1383,7 @@ extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t n
             if (e != hipSuccess) return e;
             const size_t ntile = (size_t)BC_TILES_PER_ROW * nr;
             const unsigned grid = (unsigned)std::min<size_t>(ntile, 2 * (size_t)std::max(ncu, 1));
-            hipLaunchKernelGGL(fourier_step_bc_k, dim3(grid), dim3(256), 0, stream, Sr, sp, m, (uint32_t)nr, twL,
+            hipLaunchKernelGGL(fourier_step_bc_k, dim3(grid), dim3(BC_NT), 0, stream, Sr, sp, m, (uint32_t)nr, twL,
                                twN, out + r0 * out_stride, out_stride);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
