@@ -309,7 +309,12 @@ __global__ __launch_bounds__(NT) void fdd_forward_k(
         return (wsrc[t >> lspw] >> ((uint32_t)(t & ((1u << lspw) - 1u)) * nbits)) & mask;
     };
     auto sample = [&](size_t t) -> float {
-        if (period) t %= period;
+        if (period) {
+            // t < 2 period for every chunk of a series at least one segment
+            // long: a subtraction, the 64-bit remainder only beyond that
+            if (t >= period) t -= period;
+            if (t >= period) t %= period;
+        }
         return (t < nvalid) ? ((float)value(t) - mid) * s : 0.f;
     };
     auto load = [&](int n) -> float2 {
@@ -739,7 +744,12 @@ __global__ __launch_bounds__(1024) void split_fwd_k(
         return (wsrc[t >> lspw] >> ((uint32_t)(t & ((1u << lspw) - 1u)) * nbits)) & mask;
     };
     auto sample = [&](size_t t) -> float {
-        if (period) t %= period;
+        if (period) {
+            // t < 2 period for every chunk of a series at least one segment
+            // long: a subtraction, the 64-bit remainder only beyond that
+            if (t >= period) t -= period;
+            if (t >= period) t %= period;
+        }
         return (t < nvalid) ? ((float)value(t) - mid) * s : 0.f;
     };
     // packed point r + R*n: samples t0 + 2(r + R n) and the next
