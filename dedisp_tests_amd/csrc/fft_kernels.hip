@@ -1165,7 +1165,7 @@ __global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict
     out[row * out_stride + k] = make_float2(e.x + r.x, e.y + r.y);
 }
 
-// Steps B and C in one pass for even m <= 72. A tile is columns A = 64w + l
+// Steps B and C in one pass for even m <= 144. A tile is columns A = 64w + l
 // and their mirrors B = 8192 - A (l < 64, all m rows) of one row: the
 // workgroup stages them in LDS, and each of its BC_WAVES waves sums BC_KH of the
 // radix-2 outputs k (E[k] / O[k] over the even / odd rows, as step B) for
@@ -1178,26 +1178,27 @@ __global__ __launch_bounds__(256) void fourier_step_c_k(const float2* __restrict
 // never goes to HBM. The grid is persistent (tiles t = blockIdx.x + i
 // gridDim.x, w = t % 65, row = t / 65): the next tile's loads are in flight
 // in registers while the current one is summed.
+// Two shapes: BC_MAXM = 72 with 5 outputs per wave (LDS of two workgroups
+// per CU) and 144 with 9 (one workgroup per CU).
 constexpr int BC_WAVES = 8;
-constexpr int BC_KH = 5;                     // outputs k per wave: h = m/2 <= BC_WAVES * BC_KH
-constexpr int BC_MAXM = 72;                  // LDS of two workgroups per CU
 constexpr int BC_NT = BC_WAVES * 64;
-constexpr int BC_RPT = BC_MAXM * 128 / BC_NT;  // staged values per thread
-constexpr int BC_LDS = 128 + BC_MAXM + BC_MAXM * 128;  // float2: W_m^e, W_N^{8192 k1}, [row][128] column values
 constexpr int BC_TILES_PER_ROW = 65;
-static_assert(BC_MAXM / 2 <= BC_WAVES * BC_KH && BC_NT % 128 == 0 && BC_MAXM % (BC_NT / 128) == 0,
-              "the waves cover h = m/2 outputs, the threads the staged rows");
 
+template <int BC_MAXM, int BC_KH>
 __global__ __launch_bounds__(BC_NT) void fourier_step_bc_k(const float2* __restrict__ A, size_t a_stride,
                                                          size_t m, uint32_t nrows,
                                                          const float2* __restrict__ twL,
                                                          const float2* __restrict__ twN, float2* __restrict__ out,
                                                          size_t out_stride)
 {
+    constexpr int BC_RPT = BC_MAXM * 128 / BC_NT;              // staged values per thread
+    constexpr int BC_LDS = 2 * BC_MAXM + BC_MAXM * 128;        // float2: W_m^e, W_N^{8192 k1}, [row][128] values
+    static_assert(BC_MAXM / 2 <= BC_WAVES * BC_KH && BC_NT % 128 == 0 && BC_MAXM % (BC_NT / 128) == 0,
+                  "the waves cover h = m/2 outputs, the threads the staged rows");
     __shared__ float2 bc_lds[BC_LDS];
     float2* const wt = bc_lds;                   // W_m^e, e < m
-    float2* const wk = bc_lds + 128;             // W_N^{8192 k1}, k1 < m
-    float2* const col = bc_lds + 128 + BC_MAXM;  // [row][128]: lane l's column A at l, B at 64 + l
+    float2* const wk = bc_lds + BC_MAXM;         // W_N^{8192 k1}, k1 < m
+    float2* const col = bc_lds + 2 * BC_MAXM;    // [row][128]: lane l's column A at l, B at 64 + l
     const uint32_t h = (uint32_t)(m / 2);
     const int t = threadIdx.x, lane = t & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -1308,12 +1309,12 @@ __global__ __launch_bounds__(BC_NT) void fourier_step_bc_k(const float2* __restr
 }
 
 // Column 0 of steps B and C: the m-point DFT of the column as direct sums,
-// then row k1 paired with row (m - k1) mod m.
-__global__ __launch_bounds__(128) void fourier_col0_k(const float2* __restrict__ A, size_t a_stride, size_t m,
+// then row k1 paired with row (m - k1) mod m (m <= 256, one thread per row).
+__global__ __launch_bounds__(256) void fourier_col0_k(const float2* __restrict__ A, size_t a_stride, size_t m,
                                                       const float2* __restrict__ twL, const float2* __restrict__ twN,
                                                       float2* __restrict__ out, size_t out_stride)
 {
-    __shared__ float2 zs[128], wt[128], av[128];
+    __shared__ float2 zs[256], wt[256], av[256];
     const size_t row = blockIdx.x;
     const uint32_t k1 = threadIdx.x;
     if (k1 < m) {
@@ -1347,7 +1348,7 @@ __global__ __launch_bounds__(128) void fourier_col0_k(const float2* __restrict__
 
 }  // namespace
 
-// Whether steps B and C run fused: even m <= 72, whose h = m/2 outputs the
+// Whether steps B and C run fused: even m <= 144, whose h = m/2 outputs the
 // waves of fourier_step_bc_k cover (DEDISP_FDD_BC=0 keeps the two passes).
 inline bool bc_fused(size_t m)
 {
@@ -1355,7 +1356,7 @@ inline bool bc_fused(size_t m)
         const char* v = std::getenv("DEDISP_FDD_BC");
         return v && v[0] == '0';
     }();
-    return !off && m % 2 == 0 && m <= (size_t)BC_MAXM;
+    return !off && m % 2 == 0 && m <= 144;
 }
 
 extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t nrows, size_t N,
@@ -1386,18 +1387,24 @@ extern "C" hipError_t dd_fdd_rfft_rows(const float* S, size_t s_stride, size_t n
         if (e != hipSuccess) return e;
         if (bc_fused(m)) {
             // steps B and C in one pass (column 0 on its own)
-            // persistent: two workgroups per CU (the LDS tile allows two)
+            // persistent: two workgroups per CU for m <= 72 (the LDS tile
+            // allows two), one for m <= 144
             int dev = 0, ncu = 0;
             e = hipGetDevice(&dev);
             if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
             if (e != hipSuccess) return e;
             const size_t ntile = (size_t)BC_TILES_PER_ROW * nr;
-            const unsigned grid = (unsigned)std::min<size_t>(ntile, 2 * (size_t)std::max(ncu, 1));
-            hipLaunchKernelGGL(fourier_step_bc_k, dim3(grid), dim3(BC_NT), 0, stream, Sr, sp, m, (uint32_t)nr, twL,
-                               twN, out + r0 * out_stride, out_stride);
+            const bool small = m <= 72;
+            const unsigned grid = (unsigned)std::min<size_t>(ntile, (small ? 2 : 1) * (size_t)std::max(ncu, 1));
+            if (small)
+                hipLaunchKernelGGL((fourier_step_bc_k<72, 5>), dim3(grid), dim3(BC_NT), 0, stream, Sr, sp, m,
+                                   (uint32_t)nr, twL, twN, out + r0 * out_stride, out_stride);
+            else
+                hipLaunchKernelGGL((fourier_step_bc_k<144, 9>), dim3(grid), dim3(BC_NT), 0, stream, Sr, sp, m,
+                                   (uint32_t)nr, twL, twN, out + r0 * out_stride, out_stride);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(fourier_col0_k, dim3(nr), dim3(128), 0, stream, Sr, sp, m, twL, twN,
+            hipLaunchKernelGGL(fourier_col0_k, dim3(nr), dim3(256), 0, stream, Sr, sp, m, twL, twN,
                                out + r0 * out_stride, out_stride);
             e = hipGetLastError();
             if (e != hipSuccess) return e;

@@ -212,13 +212,15 @@ def test_fdd_fourier_golden(dd):
 
 
 @pytest.mark.parametrize("nsamps,nch", [(40000, 64), (65536, 256), (2 ** 19, 1024), (1090000, 16),
-                                         (16000, 16), (1160000, 16), (1200000, 16)])
+                                         (16000, 16), (1160000, 16), (1200000, 16), (2350000, 16),
+                                         (2410000, 16)])
 def test_fdd_fourier_vs_oracle(dd, orc, nsamps, nch):
     """The whole-series Fourier output (the fork's default) against the oracle;
     the row transforms' m-point step (N = 16384 m) runs for odd m (3, 5, 67: the
-    direct DFT, 67 prime), for even m <= 72 fused with the real-input pairing
-    (2, 34, 72: one radix-2 split, mirror columns paired in LDS) and for even
-    m > 72 as two passes (74)."""
+    direct DFT, 67 prime), for even m <= 144 fused with the real-input pairing
+    (one radix-2 split, mirror columns paired in LDS: 2, 34, 72 in the two
+    workgroups per CU shape, 74, 144 in the one per CU shape) and for even
+    m > 144 as two passes (148)."""
     rng = np.random.default_rng(nsamps)
     dt, f0, df = 54.6e-6, 1530.0, -0.3125 * 1024 / nch
     data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch)).astype(np.float32)), 0, 255).astype(np.uint8)
@@ -232,6 +234,26 @@ def test_fdd_fourier_vs_oracle(dd, orc, nsamps, nch):
     ref = orc.fdd_fourier(nch, delays, dms, nsamps, plan.max_delay, data)
     assert out.shape == ref.shape
     assert rel_l2(out, ref) < REL_FP32
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_fdd_fourier_matrix_forms_vs_oracle(dd, orc, prec):
+    """The whole-series Fourier output with the bfloat16 and float16
+    contraction forms (packed spectra, mirror bins, bin-block layout; m = 34,
+    so the fused m-point step) against the oracle, at the forms' tolerances."""
+    nsamps, nch = 2 ** 19, 1024
+    rng = np.random.default_rng(nsamps + 7)
+    dt, f0, df = 54.6e-6, 1530.0, -0.3125
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch)).astype(np.float32)), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.array([0.0, 37.3, 511.0, 1023.5], f32)
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_precision({"bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec])
+    plan.set_dm_list(dms)
+    out = plan.execute(data, 8, 32)
+    ref = orc.fdd_fourier(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < PREC_TOL[prec]
 
 
 @pytest.mark.parametrize("nbits", [2, 4])
