@@ -1209,7 +1209,7 @@ __global__ __launch_bounds__(BC_NT) void fourier_step_bc_k(const float2* __restr
     // (RS = BC_NT / 128 rows a step), i < BC_RPT, the rows past m - 1 clamped
     // to m - 1 (loaded and stored, never summed); columns clamped into
     // [1, 4096] (the lanes past them compute but do not write). Byte offsets
-    // from the row base fit 32 bits (< 72 x 8192 x 8)
+    // from the row base fit 32 bits (< BC_MAXM x 8192 x 8 <= 144 x 8192 x 8)
     constexpr uint32_t RS = BC_NT / 128;
     const uint32_t sc = t & 127, sr0 = t >> 7;
     float2 v[BC_RPT];
