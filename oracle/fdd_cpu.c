@@ -9,7 +9,8 @@
  * dedisperse_optimized<float,float,32,false> (:277-403), transpose_data and
  * copy_data (src/fdd/helper.h:18-60), the spin-frequency table (:934-946) and
  * the FFTW wrappers (:949-1015). Its semantics are kept as the reference has
- * them: phasors use f_k = k/(nsamp*dt) and fractional delays
+ * them, including where they differ from the GPU path: phasors use
+ * f_k = k/(nsamp*dt) and fractional delays
  * dm*delay*dt while the FFT length is round_up(nsamp+1, 16384); only bins
  * 0..nsamp/2 are filled. The result is therefore not the dedispersed series
  * of the GPU path; it is a timing baseline.
