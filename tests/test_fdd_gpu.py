@@ -117,6 +117,42 @@ def test_fdd_device_pointers(dd, orc):
     assert rel_l2(d_out.cpu().numpy(), ref) < REL_FP32
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("fourier", [False, True])
+def test_fdd_outputs_stay_inside_the_buffer(dd, orc, prec, fourier):
+    """Device-pointer outputs with guard regions before and after them: the
+    chunk inverses (stores bounded by a buffer record count, the last chunk
+    of every row partly valid) and the row transforms write nothing outside
+    the rows, in every contraction form and both output modes."""
+    import torch
+    rng = np.random.default_rng(81)
+    nch, nsamps = 128, 50000
+    dt, f0, df = 64e-6, 1581.8, -0.78125
+    data = np.clip(np.rint(rng.normal(128, 20, size=(nsamps, nch))), 0, 255).astype(np.uint8)
+    delays = orc.delay_table(nch, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 300, 40, dtype=f32)
+    plan = dd.Plan(nch, dt, f0, df, impl=dd.FDD)
+    plan.fdd_set_output(dd.FDD_FOURIER if fourier else dd.FDD_TIMESERIES)
+    plan.fdd_set_precision({"fp32": dd.FDD_FP32, "bf16": dd.FDD_BF16, "fp16": dd.FDD_FP16}[prec])
+    plan.set_dm_list(dms)
+    n_out = plan.out_length(nsamps)
+    guard, sentinel = 8192, 12345.5
+    buf = torch.full((2 * guard + dms.size * n_out,), sentinel, dtype=torch.float32, device="cuda")
+    d_out = buf[guard:guard + dms.size * n_out].view(dms.size, n_out)
+    d_in = torch.from_numpy(data).cuda()
+    torch.cuda.synchronize()
+    assert plan.execute_raw(nsamps, d_in, 8, d_out, 32, dd.DEVICE_POINTERS) == 0
+    torch.cuda.synchronize()
+    host = buf.cpu().numpy()
+    assert np.all(host[:guard] == sentinel) and np.all(host[guard + dms.size * n_out:] == sentinel)
+    out = host[guard:guard + dms.size * n_out].reshape(dms.size, n_out)
+    if fourier:
+        ref = orc.fdd_fourier(nch, delays, dms, nsamps, plan.max_delay, data)
+    else:
+        ref = orc.fdd_timeseries(nch, delays, dms, nsamps, plan.max_delay, data)
+    assert rel_l2(out, ref) < PREC_TOL[prec]
+
+
 def test_fdd_validation(dd):
     plan = _fdd_plan(dd, 64, 64e-6, 1581.8, -2.0)
     buf = np.zeros((1000, 64), np.uint8)
